@@ -1,0 +1,193 @@
+/*
+ * ntt_amd.h — C ABI of the MI355X (gfx950) RNS multi-limb negacyclic NTT/INTT.
+ *
+ * Plain C types only (pointers, sizes, integers).  Device pointers are HIP
+ * device addresses; `stream` is a hipStream_t passed as void* (NULL = the
+ * legacy default stream, which is what the reference launches on).
+ *
+ * Data layout (same as the reference): operand A[limb * N + i], uint32,
+ * N = 2^logN, transformed in place.  Forward: natural-order input,
+ * bit-reversed-order output.  Inverse: bit-reversed input, natural output.
+ *   Montgomery variant (src/v04_montgomery.cu): forward input normal domain,
+ *     output Montgomery form (x * 2^32 mod q); inverse input Montgomery form,
+ *     output normal domain after the N^-1 scaling.
+ *   Plain variant (src/v03_template_config.cu): normal domain throughout.
+ * Inputs are expected in [0, q); outputs are always canonical, in [0, q).
+ *
+ * Errors: every entry point returns an ntt_status.  Launch entry points are
+ * asynchronous on `stream`; a kernel fault surfaces at the caller's next
+ * synchronisation, as with the reference (src/v04_montgomery.cu:990-991).
+ * Where the reference calls std::exit (src/v04_montgomery.cu:685-688,
+ * 702-706, 787-790) this library returns a status instead.
+ */
+#ifndef NTT_AMD_H
+#define NTT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    NTT_OK = 0,
+    NTT_ERR_INVALID_ARGUMENT = 1,   /* null pointer, logN out of range, zero limbs */
+    NTT_ERR_MODULUS_CONGRUENCE = 2, /* q != 1 mod 2N            (v04:685-688) */
+    NTT_ERR_PRIMITIVE_ROOT = 3,     /* root does not give a primitive 2N-th root (v04:702-706) */
+    NTT_ERR_MODULUS_RANGE = 4,      /* q even, q <= 1 or q >= 2^31 (Montgomery bound, v04:110-118) */
+    NTT_ERR_CONFIG = 5,             /* phase config inconsistent with logN or the device */
+    NTT_ERR_HIP = 6,                /* a HIP runtime call failed */
+    NTT_ERR_OUT_OF_MEMORY = 7,
+    NTT_ERR_BUDGET = 8,             /* request larger than the server's HBM budget */
+    NTT_ERR_NOT_FOUND = 9,          /* unknown request id */
+    NTT_ERR_BUSY = 10,              /* queue full / try again */
+    NTT_ERR_NO_DEVICE = 11
+} ntt_status;
+
+typedef enum { NTT_DOMAIN_PLAIN = 0, NTT_DOMAIN_MONTGOMERY = 1 } ntt_domain;
+typedef enum { NTT_FORWARD = 0, NTT_INVERSE = 1 } ntt_direction;
+
+/* ModulusConfig, src/ntt_config.h:6-9 */
+typedef struct {
+    uint32_t qi;
+    uint32_t primitive_root;
+} ntt_modulus_config;
+
+/* PhaseConfig, src/ntt_config.h:12-16 (all log2).  On gfx950:
+ *   radix_stages   stages of the transform done by this phase;
+ *   stage_merging  stages done per register round (2^S words per thread);
+ *   warp_batching  column phase: log2 of consecutive columns per tile
+ *                  (6 = one wave64 covers 64 columns, 256 B per row);
+ *                  row phase: log2 of rows per workgroup.
+ * Workgroup = 2^(radix_stages - stage_merging + warp_batching) threads,
+ * LDS = workgroup << stage_merging words, as in src/v04_montgomery.cu:582-601. */
+typedef struct {
+    int32_t radix_stages;
+    int32_t stage_merging;
+    int32_t warp_batching;
+} ntt_phase_config;
+
+/* TransformConfig, src/ntt_config.h:18-23 */
+typedef struct {
+    ntt_phase_config ntt_phase1;   /* forward column phase (outer stages) */
+    ntt_phase_config ntt_phase2;   /* forward row phase (inner stages)    */
+    ntt_phase_config intt_phase1;  /* inverse row phase (inner stages)    */
+    ntt_phase_config intt_phase2;  /* inverse column phase + N^-1 scaling */
+} ntt_transform_config;
+
+/* ModulusRuntime, src/v04_montgomery.cu:17-26 */
+typedef struct {
+    uint32_t qi, primitive_root, q_inv, r2, psi, psi_inv, omega, degree_inv;
+} ntt_modulus_runtime;
+
+/* ---------------------------------------------------------------------------
+ * Library / device information
+ * ------------------------------------------------------------------------- */
+const char* ntt_status_string(int status);
+const char* ntt_version(void);
+/* Max logN supported by the kernels (17 for the default phase split, up to 20). */
+int ntt_max_log_n(void);
+
+/* ---------------------------------------------------------------------------
+ * Configuration API — src/ntt_config.h
+ * ------------------------------------------------------------------------- */
+/* default_moduli(), src/ntt_config.h:75-94.  Writes up to `cap` entries and
+ * returns the number available (16). */
+int ntt_default_moduli(ntt_modulus_config* out, int cap);
+
+/* DefaultNTTLaunchConfig<LogN>, src/ntt_config.h:62-73, with warp_batching
+ * re-derived for wave64 (column phases 6, row phases 0). */
+int ntt_default_transform_config(int logN, ntt_transform_config* out);
+
+/* The reference's literal DefaultNTTLaunchConfig<LogN> values (W = 4 / 0). */
+int ntt_reference_transform_config(int logN, ntt_transform_config* out);
+
+/* Checks what the reference leaves unchecked (radix1 + radix2 == logN,
+ * stage_merging <= radix_stages, 2^S <= 32, workgroup <= 1024, LDS fits,
+ * column tile <= columns).  Returns NTT_OK or NTT_ERR_CONFIG. */
+int ntt_validate_transform_config(int logN, const ntt_transform_config* cfg);
+
+/* The `count` largest primes q < bound with q = 1 (mod 2^(logN+1)), in
+ * descending order, each with its smallest primitive root.  Returns the number
+ * written (fewer if the range runs out). */
+int ntt_generate_moduli(int logN, uint32_t bound, int count, ntt_modulus_config* out);
+
+/* ---------------------------------------------------------------------------
+ * Host precompute — src/v04_montgomery.cu:125-141, 660-740
+ * ------------------------------------------------------------------------- */
+int ntt_make_modulus_runtime(int logN, ntt_modulus_config cfg, ntt_modulus_runtime* out);
+/* tw[i] = psi^brv(i, logN) mod q, normal domain (v04:720-727). */
+int ntt_make_twiddle_table(int logN, uint32_t q, uint32_t psi, uint32_t* tw_out);
+/* Limb-major Montgomery-form tables for `limbs` moduli, as the reference's
+ * main() builds them (v04:893-915).  Any output pointer may be NULL. */
+int ntt_make_device_tables_host(int logN, const ntt_modulus_config* moduli, size_t limbs,
+                                int domain, uint32_t* twiddles, uint32_t* inv_twiddles,
+                                uint32_t* moduli_out, uint32_t* q_inv, uint32_t* r2,
+                                uint32_t* degree_inv);
+uint32_t ntt_mont_q_inv(uint32_t q);           /* v04:125-131 */
+uint32_t ntt_to_mont(uint32_t x, uint32_t q);  /* v04:133-136 */
+uint32_t ntt_mont_r2(uint32_t q);              /* v04:138-141 */
+uint32_t ntt_from_mont(uint32_t x, uint32_t q, uint32_t q_inv); /* v04:143-146 */
+
+/* ---------------------------------------------------------------------------
+ * Drop-in launchers.  Same argument meaning as the reference's; logN and the
+ * phase config (NULL = ntt_default_transform_config) replace the template
+ * parameter LaunchConfig.
+ * ------------------------------------------------------------------------- */
+/* launch_ntt<LC>, src/v04_montgomery.cu:603-626 (Montgomery-form twiddles,
+ * fused x R^2 input conversion). */
+int ntt_launch_ntt_montgomery(uint32_t* A_dev, const uint32_t* twiddle_dev,
+                              const uint32_t* moduli_dev, const uint32_t* q_inv_dev,
+                              const uint32_t* r2_dev, size_t limb_count, int logN,
+                              const ntt_transform_config* cfg, void* stream);
+/* launch_intt<LC>, src/v04_montgomery.cu:628-658 (degree_inv_dev holds
+ * to_mont(N^-1); NormalizeKernel is fused into the last phase). */
+int ntt_launch_intt_montgomery(uint32_t* A_dev, const uint32_t* inv_twiddle_dev,
+                               const uint32_t* moduli_dev, const uint32_t* q_inv_dev,
+                               const uint32_t* degree_inv_dev, size_t limb_count, int logN,
+                               const ntt_transform_config* cfg, void* stream);
+/* launch_ntt<LC>, src/v03_template_config.cu:549-571 (normal-domain twiddles). */
+int ntt_launch_ntt_plain(uint32_t* A_dev, const uint32_t* twiddle_dev,
+                         const uint32_t* moduli_dev, size_t limb_count, int logN,
+                         const ntt_transform_config* cfg, void* stream);
+/* launch_intt<LC>, src/v03_template_config.cu:573-603 (degree_inv_dev normal domain). */
+int ntt_launch_intt_plain(uint32_t* A_dev, const uint32_t* inv_twiddle_dev,
+                          const uint32_t* moduli_dev, const uint32_t* degree_inv_dev,
+                          size_t limb_count, int logN, const ntt_transform_config* cfg,
+                          void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Context + plans: device-resident Montgomery twiddle cache keyed by (q, logN),
+ * replacing the per-call table setup in the reference's main()
+ * (src/v04_montgomery.cu:882-916, 947-967).
+ * ------------------------------------------------------------------------- */
+typedef struct ntt_context ntt_context;
+typedef struct ntt_plan ntt_plan;
+
+int ntt_context_create(int device, ntt_context** out);
+int ntt_context_destroy(ntt_context* ctx);
+/* Bytes of device memory held by the twiddle cache. */
+size_t ntt_context_twiddle_bytes(const ntt_context* ctx);
+/* Number of distinct (q, logN) tables in the cache. */
+size_t ntt_context_twiddle_entries(const ntt_context* ctx);
+
+/* cfg NULL = the autotuned config for (logN, limbs) if one is loaded, else
+ * the default.  Validates every modulus (the checks the reference exits on). */
+int ntt_plan_create(ntt_context* ctx, int logN, const ntt_modulus_config* moduli,
+                    size_t limbs, int domain, const ntt_transform_config* cfg,
+                    ntt_plan** out);
+int ntt_plan_destroy(ntt_plan* plan);
+int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out);
+int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream);
+int ntt_plan_inverse(ntt_plan* plan, uint32_t* A_dev, void* stream);
+/* Same transform over `count` operands laid out back to back
+ * (operand k at A_dev + k * limbs * N) in one launch per phase. */
+int ntt_plan_forward_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream);
+int ntt_plan_inverse_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NTT_AMD_H */

@@ -1,0 +1,413 @@
+"""ntt_amd — MI355X (gfx950) RNS multi-limb negacyclic NTT/INTT.
+
+Python mirror of the reference's host API over the library's C ABI
+(include/ntt_amd.h).  Names follow the reference:
+
+  ModulusConfig, PhaseConfig, TransformConfig  src/ntt_config.h:6-23
+  default_moduli()                              src/ntt_config.h:75-94
+  default_launch_config(logN)                   DefaultNTTLaunchConfig<LogN>, src/ntt_config.h:62-73
+  launch_ntt / launch_intt                      src/v04_montgomery.cu:603-658 (Montgomery)
+  launch_ntt_plain / launch_intt_plain          src/v03_template_config.cu:549-603
+  make_modulus_runtime, make_twiddle_table      src/v04_montgomery.cu:681-727
+
+Device buffers are passed as torch tensors (any 32-bit dtype; the bits are
+read as uint32) or as raw integer device addresses.  The HIP library is
+required: importing this package raises if libntt_amd.so is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from dataclasses import dataclass
+from typing import Iterable, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libntt_amd.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "ntt_amd.h")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"ntt_amd: HIP library not built ({LIB_PATH} missing); run `python -m ntt_amd.build`")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+NTT_OK = 0
+STATUS = {
+    0: "NTT_OK", 1: "NTT_ERR_INVALID_ARGUMENT", 2: "NTT_ERR_MODULUS_CONGRUENCE",
+    3: "NTT_ERR_PRIMITIVE_ROOT", 4: "NTT_ERR_MODULUS_RANGE", 5: "NTT_ERR_CONFIG",
+    6: "NTT_ERR_HIP", 7: "NTT_ERR_OUT_OF_MEMORY", 8: "NTT_ERR_BUDGET", 9: "NTT_ERR_NOT_FOUND",
+    10: "NTT_ERR_BUSY", 11: "NTT_ERR_NO_DEVICE",
+}
+PLAIN, MONTGOMERY = 0, 1
+FORWARD, INVERSE = 0, 1
+
+
+class NTTError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = _lib.ntt_status_string(status).decode()
+        super().__init__(f"{what}: {STATUS.get(status, status)} ({msg})" if what else msg)
+
+
+# ---- C structs --------------------------------------------------------------
+class ModulusConfig(ctypes.Structure):
+    """ModulusConfig{qi, primitive_root}, src/ntt_config.h:6-9."""
+    _fields_ = [("qi", ctypes.c_uint32), ("primitive_root", ctypes.c_uint32)]
+
+    def __repr__(self):
+        return f"ModulusConfig({self.qi}, {self.primitive_root})"
+
+    def __iter__(self):
+        yield self.qi
+        yield self.primitive_root
+
+
+class PhaseConfig(ctypes.Structure):
+    """PhaseConfig{radix_stages, stage_merging, warp_batching}, src/ntt_config.h:12-16."""
+    _fields_ = [("radix_stages", ctypes.c_int32), ("stage_merging", ctypes.c_int32),
+                ("warp_batching", ctypes.c_int32)]
+
+    def astuple(self):
+        return (self.radix_stages, self.stage_merging, self.warp_batching)
+
+    def __repr__(self):
+        return f"PhaseConfig{self.astuple()}"
+
+
+class TransformConfig(ctypes.Structure):
+    """TransformConfig, src/ntt_config.h:18-23."""
+    _fields_ = [("ntt_phase1", PhaseConfig), ("ntt_phase2", PhaseConfig),
+                ("intt_phase1", PhaseConfig), ("intt_phase2", PhaseConfig)]
+
+    @classmethod
+    def make(cls, p1, p2, ip1=None, ip2=None):
+        ip1 = p2 if ip1 is None else ip1
+        ip2 = p1 if ip2 is None else ip2
+        return cls(PhaseConfig(*p1), PhaseConfig(*p2), PhaseConfig(*ip1), PhaseConfig(*ip2))
+
+    def astuple(self):
+        return (self.ntt_phase1.astuple(), self.ntt_phase2.astuple(),
+                self.intt_phase1.astuple(), self.intt_phase2.astuple())
+
+    def __repr__(self):
+        return f"TransformConfig{self.astuple()}"
+
+
+class ModulusRuntime(ctypes.Structure):
+    """ModulusRuntime, src/v04_montgomery.cu:17-26."""
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("qi", "primitive_root", "q_inv", "r2", "psi", "psi_inv", "omega", "degree_inv")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+@dataclass
+class VersionConfig:
+    """VersionConfig{logN, moduli, transform}, src/ntt_config.h:25-29."""
+    logN: int
+    moduli: List[ModulusConfig]
+    transform: TransformConfig
+
+
+# ---- prototypes ---------------------------------------------------------------
+_P = ctypes.c_void_p
+_U32 = ctypes.c_uint32
+_SZ = ctypes.c_size_t
+_I = ctypes.c_int
+_PU32 = ctypes.POINTER(ctypes.c_uint32)
+_PCFG = ctypes.POINTER(TransformConfig)
+
+_PROTOS = {
+    "ntt_status_string": (ctypes.c_char_p, [_I]),
+    "ntt_version": (ctypes.c_char_p, []),
+    "ntt_max_log_n": (_I, []),
+    "ntt_default_moduli": (_I, [ctypes.POINTER(ModulusConfig), _I]),
+    "ntt_default_transform_config": (_I, [_I, _PCFG]),
+    "ntt_reference_transform_config": (_I, [_I, _PCFG]),
+    "ntt_validate_transform_config": (_I, [_I, _PCFG]),
+    "ntt_generate_moduli": (_I, [_I, _U32, _I, ctypes.POINTER(ModulusConfig)]),
+    "ntt_make_modulus_runtime": (_I, [_I, ModulusConfig, ctypes.POINTER(ModulusRuntime)]),
+    "ntt_make_twiddle_table": (_I, [_I, _U32, _U32, _PU32]),
+    "ntt_make_device_tables_host": (_I, [_I, ctypes.POINTER(ModulusConfig), _SZ, _I, _PU32, _PU32, _PU32,
+                                         _PU32, _PU32, _PU32]),
+    "ntt_mont_q_inv": (_U32, [_U32]),
+    "ntt_to_mont": (_U32, [_U32, _U32]),
+    "ntt_mont_r2": (_U32, [_U32]),
+    "ntt_from_mont": (_U32, [_U32, _U32, _U32]),
+    "ntt_launch_ntt_montgomery": (_I, [_P, _P, _P, _P, _P, _SZ, _I, _PCFG, _P]),
+    "ntt_launch_intt_montgomery": (_I, [_P, _P, _P, _P, _P, _SZ, _I, _PCFG, _P]),
+    "ntt_launch_ntt_plain": (_I, [_P, _P, _P, _SZ, _I, _PCFG, _P]),
+    "ntt_launch_intt_plain": (_I, [_P, _P, _P, _P, _SZ, _I, _PCFG, _P]),
+    "ntt_context_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "ntt_context_destroy": (_I, [_P]),
+    "ntt_context_twiddle_bytes": (_SZ, [_P]),
+    "ntt_context_twiddle_entries": (_SZ, [_P]),
+    "ntt_plan_create": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _I, _PCFG, ctypes.POINTER(_P)]),
+    "ntt_plan_destroy": (_I, [_P]),
+    "ntt_plan_get_config": (_I, [_P, _PCFG]),
+    "ntt_plan_forward": (_I, [_P, _P, _P]),
+    "ntt_plan_inverse": (_I, [_P, _P, _P]),
+    "ntt_plan_forward_many": (_I, [_P, _P, _SZ, _P]),
+    "ntt_plan_inverse_many": (_I, [_P, _P, _SZ, _P]),
+}
+
+
+def _bind(lib):
+    for name, (res, args) in _PROTOS.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+_bind(_lib)
+
+
+def lib():
+    return _lib
+
+
+def header_functions(path: str = HEADER) -> List[str]:
+    """Names of every function declared in include/*.h."""
+    names = []
+    inc = os.path.dirname(path)
+    for f in sorted(os.listdir(inc)):
+        if f.endswith(".h"):
+            txt = open(os.path.join(inc, f)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names += re.findall(r"\b(ntt_[a-z0-9_]+)\s*\(", txt)
+    return sorted(set(names))
+
+
+def _check(rc: int, what: str):
+    if rc != NTT_OK:
+        raise NTTError(rc, what)
+
+
+def _addr(x) -> Optional[int]:
+    """Device address of a torch tensor, or an int address, or None."""
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return int(x.data_ptr())
+
+
+def _stream(stream) -> Optional[int]:
+    if stream is None:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                return int(torch.cuda.current_stream().cuda_stream) or None
+        except Exception:
+            return None
+        return None
+    if isinstance(stream, int):
+        return stream or None
+    return int(stream.cuda_stream) or None
+
+
+def _cfg_ptr(cfg: Optional[TransformConfig]):
+    return ctypes.byref(cfg) if cfg is not None else None
+
+
+# ---- configuration API ---------------------------------------------------------
+def default_moduli() -> List[ModulusConfig]:
+    n = _lib.ntt_default_moduli(None, 0)
+    arr = (ModulusConfig * n)()
+    _lib.ntt_default_moduli(arr, n)
+    return list(arr)
+
+
+def default_launch_config(logN: int) -> TransformConfig:
+    """DefaultNTTLaunchConfig<LogN> with warp_batching re-derived for wave64."""
+    c = TransformConfig()
+    _check(_lib.ntt_default_transform_config(logN, ctypes.byref(c)), "default_launch_config")
+    return c
+
+
+def reference_launch_config(logN: int) -> TransformConfig:
+    """The reference's literal DefaultNTTLaunchConfig<LogN> values."""
+    c = TransformConfig()
+    _check(_lib.ntt_reference_transform_config(logN, ctypes.byref(c)), "reference_launch_config")
+    return c
+
+
+def validate_config(logN: int, cfg: TransformConfig) -> int:
+    return _lib.ntt_validate_transform_config(logN, ctypes.byref(cfg))
+
+
+def make_version_config(logN: int = 16, moduli=None, transform=None) -> VersionConfig:
+    """make_version_config<LC>(), src/ntt_config.h:96-103."""
+    return VersionConfig(logN, list(moduli) if moduli is not None else default_moduli(),
+                         transform if transform is not None else default_launch_config(logN))
+
+
+def default_ntt_config() -> VersionConfig:
+    """default_ntt_config(), src/ntt_config.h:105-107."""
+    return make_version_config(16)
+
+
+def generate_moduli(logN: int, count: int, bound: int = 1 << 30) -> List[ModulusConfig]:
+    """The `count` largest primes q < bound with q = 1 mod 2N, smallest primitive roots."""
+    arr = (ModulusConfig * max(count, 1))()
+    n = _lib.ntt_generate_moduli(logN, bound, count, arr)
+    if n < 0:
+        raise NTTError(-n, "generate_moduli")
+    return list(arr)[:n]
+
+
+def as_moduli(moduli) -> List[ModulusConfig]:
+    return [m if isinstance(m, ModulusConfig) else ModulusConfig(int(m[0]), int(m[1])) for m in moduli]
+
+
+# ---- host precompute --------------------------------------------------------------
+def make_modulus_runtime(logN: int, modulus) -> dict:
+    m = as_moduli([modulus])[0]
+    rt = ModulusRuntime()
+    _check(_lib.ntt_make_modulus_runtime(logN, m, ctypes.byref(rt)), "make_modulus_runtime")
+    return rt.as_dict()
+
+
+def make_twiddle_table(logN: int, q: int, psi: int) -> np.ndarray:
+    out = np.empty(1 << logN, dtype=np.uint32)
+    _check(_lib.ntt_make_twiddle_table(logN, q, psi, out.ctypes.data_as(_PU32)), "make_twiddle_table")
+    return out
+
+
+@dataclass
+class DeviceTables:
+    """Host arrays laid out as the reference's main() uploads them (v04:893-915)."""
+    twiddles: np.ndarray
+    inv_twiddles: np.ndarray
+    moduli: np.ndarray
+    q_inv: np.ndarray
+    r2: np.ndarray
+    degree_inv: np.ndarray
+
+
+def make_tables(logN: int, moduli, domain: int = MONTGOMERY) -> DeviceTables:
+    ms = as_moduli(moduli)
+    L, N = len(ms), 1 << logN
+    arr = (ModulusConfig * L)(*ms)
+    t = DeviceTables(*(np.empty(L * N, np.uint32) for _ in range(2)), *(np.empty(L, np.uint32) for _ in range(4)))
+    p = lambda a: a.ctypes.data_as(_PU32)
+    _check(_lib.ntt_make_device_tables_host(logN, arr, L, domain, p(t.twiddles), p(t.inv_twiddles), p(t.moduli),
+                                            p(t.q_inv), p(t.r2), p(t.degree_inv)), "make_tables")
+    return t
+
+
+def mont_q_inv(q): return int(_lib.ntt_mont_q_inv(q))
+def to_mont(x, q): return int(_lib.ntt_to_mont(x, q))
+def mont_r2(q): return int(_lib.ntt_mont_r2(q))
+def from_mont(x, q, q_inv): return int(_lib.ntt_from_mont(x, q, q_inv))
+
+
+# ---- drop-in launchers (device pointers / tensors) ------------------------------------
+def launch_ntt(A, twiddles, moduli, q_inv, r2, limb_count: int, logN: int,
+               cfg: Optional[TransformConfig] = None, stream=None):
+    """launch_ntt<LC>, src/v04_montgomery.cu:603-626 (Montgomery variant)."""
+    _check(_lib.ntt_launch_ntt_montgomery(_addr(A), _addr(twiddles), _addr(moduli), _addr(q_inv), _addr(r2),
+                                          limb_count, logN, _cfg_ptr(cfg), _stream(stream)), "launch_ntt")
+
+
+def launch_intt(A, inv_twiddles, moduli, q_inv, degree_inv, limb_count: int, logN: int,
+                cfg: Optional[TransformConfig] = None, stream=None):
+    """launch_intt<LC>, src/v04_montgomery.cu:628-658 (Montgomery variant)."""
+    _check(_lib.ntt_launch_intt_montgomery(_addr(A), _addr(inv_twiddles), _addr(moduli), _addr(q_inv),
+                                           _addr(degree_inv), limb_count, logN, _cfg_ptr(cfg), _stream(stream)),
+           "launch_intt")
+
+
+def launch_ntt_plain(A, twiddles, moduli, limb_count: int, logN: int, cfg=None, stream=None):
+    """launch_ntt<LC>, src/v03_template_config.cu:549-571 (plain variant)."""
+    _check(_lib.ntt_launch_ntt_plain(_addr(A), _addr(twiddles), _addr(moduli), limb_count, logN,
+                                     _cfg_ptr(cfg), _stream(stream)), "launch_ntt_plain")
+
+
+def launch_intt_plain(A, inv_twiddles, moduli, degree_inv, limb_count: int, logN: int, cfg=None, stream=None):
+    """launch_intt<LC>, src/v03_template_config.cu:573-603 (plain variant)."""
+    _check(_lib.ntt_launch_intt_plain(_addr(A), _addr(inv_twiddles), _addr(moduli), _addr(degree_inv),
+                                      limb_count, logN, _cfg_ptr(cfg), _stream(stream)), "launch_intt_plain")
+
+
+# ---- context / plans ----------------------------------------------------------------
+class Context:
+    """Device context owning the (q, logN, direction) twiddle cache."""
+
+    def __init__(self, device: int = 0):
+        h = ctypes.c_void_p()
+        _check(_lib.ntt_context_create(device, ctypes.byref(h)), "ntt_context_create")
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            _lib.ntt_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def twiddle_bytes(self) -> int:
+        return int(_lib.ntt_context_twiddle_bytes(self.handle))
+
+    @property
+    def twiddle_entries(self) -> int:
+        return int(_lib.ntt_context_twiddle_entries(self.handle))
+
+    def plan(self, logN: int, moduli, domain: int = MONTGOMERY, cfg: Optional[TransformConfig] = None) -> "Plan":
+        return Plan(self, logN, moduli, domain, cfg)
+
+
+class Plan:
+    def __init__(self, ctx: Context, logN: int, moduli, domain: int = MONTGOMERY,
+                 cfg: Optional[TransformConfig] = None):
+        ms = as_moduli(moduli)
+        arr = (ModulusConfig * len(ms))(*ms)
+        h = ctypes.c_void_p()
+        _check(_lib.ntt_plan_create(ctx.handle, logN, arr, len(ms), domain, _cfg_ptr(cfg), ctypes.byref(h)),
+               "ntt_plan_create")
+        self.ctx, self.handle, self.logN, self.limbs, self.domain = ctx, h, logN, len(ms), domain
+        self.moduli = ms
+
+    @property
+    def config(self) -> TransformConfig:
+        c = TransformConfig()
+        _check(_lib.ntt_plan_get_config(self.handle, ctypes.byref(c)), "ntt_plan_get_config")
+        return c
+
+    def forward(self, A, count: int = 1, stream=None):
+        _check(_lib.ntt_plan_forward_many(self.handle, _addr(A), count, _stream(stream)), "ntt_plan_forward")
+
+    def inverse(self, A, count: int = 1, stream=None):
+        _check(_lib.ntt_plan_inverse_many(self.handle, _addr(A), count, _stream(stream)), "ntt_plan_inverse")
+
+    def close(self):
+        if self.handle:
+            _lib.ntt_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- metrics (src/v04_montgomery.cu:808-818) ---------------------------------------
+def butterflies(logN: int, limbs: int) -> float:
+    """Butterflies per transform = N/2 * logN * limbs."""
+    return 0.5 * (1 << logN) * logN * limbs
+
+
+def algorithmic_bytes(logN: int, limbs: int) -> int:
+    """Each uint32 word read once and written once."""
+    return 8 * (1 << logN) * limbs

@@ -1,0 +1,71 @@
+// Modular arithmetic for 32-bit RNS limbs on gfx950.
+//
+// Values are canonical residues in [0, q) with q odd and q < 2^31, the range
+// the reference's single-subtraction Montgomery reduction supports
+// (src/v04_montgomery.cu:110-118).  Conditional corrections use the unsigned
+// min trick (one v_sub/v_add plus one v_min_u32) instead of compare+select.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ntt {
+
+// (a + b) mod q for a, b in [0, q): a + b < 2q < 2^32; if a + b < q the
+// subtraction wraps to a value above 2^31 and min keeps a + b.
+__device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b, uint32_t q) {
+    uint32_t s = a + b;
+    return min(s, s - q);
+}
+
+// (a - b) mod q: if a < b the difference wraps above 2^31 while d + q wraps
+// back into [0, q), so min picks the right representative.
+__device__ __forceinline__ uint32_t sub_mod(uint32_t a, uint32_t b, uint32_t q) {
+    uint32_t d = a - b;
+    return min(d, d + q);
+}
+
+// Montgomery product a * b * 2^-32 mod q with q_inv = -q^-1 mod 2^32.
+// Valid for a * b < 2^32 * q (so any 32-bit a when b < q): t + m q < 2^64 and
+// the pre-correction result lies in [0, 2q).  The two 64-bit products map to
+// v_mad_u64_u32, the low product m to v_mul_lo_u32.
+__device__ __forceinline__ uint32_t mont_mul(uint32_t a, uint32_t b, uint32_t q, uint32_t q_inv) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t m = (uint32_t)t * q_inv;
+    uint32_t u = (uint32_t)(((uint64_t)m * q + t) >> 32);
+    return min(u, u - q);
+}
+
+// -q^-1 mod 2^32 by Newton iteration (q odd): each step doubles the number of
+// correct low bits, 1 -> 2 -> 4 -> ... -> 32 bits after five steps.
+__device__ __forceinline__ uint32_t neg_inv_mod_2_32(uint32_t q) {
+    uint32_t inv = 1;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) inv *= 2u - q * inv;
+    return 0u - inv;
+}
+
+// 2^64 mod q (the Montgomery R^2 constant) for kernels that are handed only q.
+__device__ __forceinline__ uint32_t r2_mod(uint32_t q) {
+    uint64_t r = (UINT64_C(1) << 32) % q;
+    return (uint32_t)((r * r) % q);
+}
+
+// Cooley-Tukey: (x, y) -> (x + w y, x - w y), w in Montgomery form.
+__device__ __forceinline__ void ct_butterfly(uint32_t& x, uint32_t& y, uint32_t w, uint32_t q,
+                                             uint32_t q_inv) {
+    uint32_t v = mont_mul(y, w, q, q_inv);
+    uint32_t u = x;
+    x = add_mod(u, v, q);
+    y = sub_mod(u, v, q);
+}
+
+// Gentleman-Sande: (x, y) -> (x + y, (x - y) w), w in Montgomery form.
+__device__ __forceinline__ void gs_butterfly(uint32_t& x, uint32_t& y, uint32_t w, uint32_t q,
+                                             uint32_t q_inv) {
+    uint32_t u = x, v = y;
+    x = add_mod(u, v, q);
+    y = mont_mul(sub_mod(u, v, q), w, q, q_inv);
+}
+
+}  // namespace ntt

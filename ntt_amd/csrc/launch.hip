@@ -1,0 +1,188 @@
+// Kernel instantiation, launch geometry and the drop-in launchers of the
+// ntt_amd C ABI.
+//
+// Reference launch geometry (src/v04_montgomery.cu:582-601): workgroup
+// 2^(R - S + W) threads, LDS (workgroup << S) words; here plus one padding
+// word per 32.  The grid is flat: limbs x operands x tiles workgroups with
+// tiles = 2^(n - R - W), so limb counts are not bounded by gridDim.y.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <mutex>
+
+#include "../../include/ntt_amd.h"
+#include "host_math.hpp"
+#include "launch.hpp"
+#include "phase_kernels.hpp"
+
+namespace ntt {
+
+using KernelFn = void (*)(PhaseParams);
+
+template <int S>
+static KernelFn pick_s(bool inv, bool col, bool twconv) {
+    if (!inv && col && !twconv) return phase_kernel<S, false, true, false>;
+    if (!inv && col && twconv) return phase_kernel<S, false, true, true>;
+    if (!inv && !col && !twconv) return phase_kernel<S, false, false, false>;
+    if (!inv && !col && twconv) return phase_kernel<S, false, false, true>;
+    if (inv && col && !twconv) return phase_kernel<S, true, true, false>;
+    if (inv && col && twconv) return phase_kernel<S, true, true, true>;
+    if (inv && !col && !twconv) return phase_kernel<S, true, false, false>;
+    return phase_kernel<S, true, false, true>;
+}
+
+static KernelFn pick(int S, bool inv, bool col, bool twconv) {
+    switch (S) {
+        case 1: return pick_s<1>(inv, col, twconv);
+        case 2: return pick_s<2>(inv, col, twconv);
+        case 3: return pick_s<3>(inv, col, twconv);
+        case 4: return pick_s<4>(inv, col, twconv);
+        case 5: return pick_s<5>(inv, col, twconv);
+        default: return nullptr;
+    }
+}
+
+static std::once_flag g_lds_once;
+static size_t g_max_lds = 64 * 1024;
+
+size_t device_max_lds() {
+    std::call_once(g_lds_once, [] {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) == hipSuccess && v > 0)
+            g_max_lds = (size_t)v;
+        if (g_max_lds > NTT_LDS_BYTES_GFX950) g_max_lds = NTT_LDS_BYTES_GFX950;
+    });
+    return g_max_lds;
+}
+
+static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, bool twconv, const PhaseParams& base,
+                       hipStream_t stream) {
+    if (pc.radix_stages == 0) return NTT_OK;
+    const int R = pc.radix_stages, S = pc.stage_merging, W = pc.warp_batching;
+    PhaseParams p = base;
+    p.R = R;
+    p.W = W;
+    p.tiles = 1u << (p.logN - R - W);
+    const size_t blocks = (size_t)p.tiles * p.limbs * p.count;
+    if (blocks == 0 || blocks > 0x7fffffffu) return NTT_ERR_INVALID_ARGUMENT;
+    p.xcd_remap = (base.xcd_remap && blocks % 8 == 0) ? 1 : 0;
+    const unsigned threads = 1u << (R - S + W);
+    const size_t lds = lds_bytes_for((size_t)1 << (R + W));
+    KernelFn fn = pick(S, inv, col, twconv);
+    if (!fn) return NTT_ERR_CONFIG;
+    if (lds > 64 * 1024) {
+        if (hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds) != hipSuccess)
+            return NTT_ERR_CONFIG;
+    }
+    hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(threads), lds, stream, p);
+    return hipGetLastError() == hipSuccess ? NTT_OK : NTT_ERR_HIP;
+}
+
+int launch_transform(const TransformLaunch& L, hipStream_t stream) {
+    if (!L.A || (!L.tw && !L.tw_ptrs) || !L.moduli || L.limbs == 0 || L.count == 0) return NTT_ERR_INVALID_ARGUMENT;
+    if (L.logN < 1 || L.logN > NTT_MAX_LOG_N) return NTT_ERR_INVALID_ARGUMENT;
+    if (L.scale_mode != SCALE_NONE && !L.scale) return NTT_ERR_INVALID_ARGUMENT;
+    if (reinterpret_cast<uintptr_t>(L.A) & 15) return NTT_ERR_INVALID_ARGUMENT;   // 16 B vector access
+    int rc = validate_config(L.logN, &L.cfg, device_max_lds());
+    if (rc) return rc;
+
+    PhaseParams p{};
+    p.A = L.A;
+    p.tw = L.tw;
+    p.tw_ptrs = L.tw_ptrs;
+    p.moduli = L.moduli;
+    p.q_inv = L.q_inv;
+    p.logN = L.logN;
+    p.limbs = (uint32_t)L.limbs;
+    p.count = (uint32_t)L.count;
+    p.xcd_remap = L.xcd_remap;
+
+    // The scale multiplies on the first load of a forward transform and on
+    // the last store of an inverse one, in whichever pass that is.
+    PhaseParams none = p;
+    none.scale = nullptr;
+    none.scale_mode = SCALE_NONE;
+    PhaseParams scaled = p;
+    scaled.scale = L.scale;
+    scaled.scale_mode = L.scale_mode;
+
+    if (!L.inverse) {
+        const ntt_phase_config &c1 = L.cfg.ntt_phase1, &c2 = L.cfg.ntt_phase2;
+        const bool first_is_col = c1.radix_stages > 0;
+        if ((rc = launch_pass(c1, false, true, L.twconv, first_is_col ? scaled : none, stream))) return rc;
+        if ((rc = launch_pass(c2, false, false, L.twconv, first_is_col ? none : scaled, stream))) return rc;
+    } else {
+        const ntt_phase_config &c1 = L.cfg.intt_phase1, &c2 = L.cfg.intt_phase2;
+        const bool last_is_col = c2.radix_stages > 0;
+        if ((rc = launch_pass(c1, true, false, L.twconv, last_is_col ? none : scaled, stream))) return rc;
+        if ((rc = launch_pass(c2, true, true, L.twconv, last_is_col ? scaled : none, stream))) return rc;
+    }
+    return NTT_OK;
+}
+
+static int resolve_cfg(int logN, const ntt_transform_config* cfg, ntt_transform_config* out) {
+    if (logN < 1 || logN > NTT_MAX_LOG_N) return NTT_ERR_INVALID_ARGUMENT;
+    if (cfg) *out = *cfg; else default_config(logN, out);
+    return NTT_OK;
+}
+
+}  // namespace ntt
+
+using namespace ntt;
+
+extern "C" {
+
+int ntt_launch_ntt_montgomery(uint32_t* A_dev, const uint32_t* twiddle_dev, const uint32_t* moduli_dev,
+                              const uint32_t* q_inv_dev, const uint32_t* r2_dev, size_t limb_count, int logN,
+                              const ntt_transform_config* cfg, void* stream) {
+    TransformLaunch L{};
+    int rc = resolve_cfg(logN, cfg, &L.cfg);
+    if (rc) return rc;
+    if (!q_inv_dev || !r2_dev) return NTT_ERR_INVALID_ARGUMENT;
+    L.A = A_dev; L.tw = twiddle_dev; L.moduli = moduli_dev; L.q_inv = q_inv_dev;
+    L.scale = r2_dev; L.scale_mode = SCALE_AS_IS;       // mont_mul(x, R^2) = x R (v04:229-232)
+    L.logN = logN; L.limbs = limb_count; L.count = 1; L.inverse = false; L.twconv = false; L.xcd_remap = 1;
+    return launch_transform(L, (hipStream_t)stream);
+}
+
+int ntt_launch_intt_montgomery(uint32_t* A_dev, const uint32_t* inv_twiddle_dev, const uint32_t* moduli_dev,
+                               const uint32_t* q_inv_dev, const uint32_t* degree_inv_dev, size_t limb_count,
+                               int logN, const ntt_transform_config* cfg, void* stream) {
+    TransformLaunch L{};
+    int rc = resolve_cfg(logN, cfg, &L.cfg);
+    if (rc) return rc;
+    if (!q_inv_dev || !degree_inv_dev) return NTT_ERR_INVALID_ARGUMENT;
+    L.A = A_dev; L.tw = inv_twiddle_dev; L.moduli = moduli_dev; L.q_inv = q_inv_dev;
+    // NormalizeKernel (v04:567-580) is from_mont(mont_mul(x, to_mont(N^-1))) = mont_mul(x, N^-1).
+    L.scale = degree_inv_dev; L.scale_mode = SCALE_FROM_MONT;
+    L.logN = logN; L.limbs = limb_count; L.count = 1; L.inverse = true; L.twconv = false; L.xcd_remap = 1;
+    return launch_transform(L, (hipStream_t)stream);
+}
+
+int ntt_launch_ntt_plain(uint32_t* A_dev, const uint32_t* twiddle_dev, const uint32_t* moduli_dev,
+                         size_t limb_count, int logN, const ntt_transform_config* cfg, void* stream) {
+    TransformLaunch L{};
+    int rc = resolve_cfg(logN, cfg, &L.cfg);
+    if (rc) return rc;
+    L.A = A_dev; L.tw = twiddle_dev; L.moduli = moduli_dev; L.q_inv = nullptr;
+    L.scale = nullptr; L.scale_mode = SCALE_NONE;
+    L.logN = logN; L.limbs = limb_count; L.count = 1; L.inverse = false; L.twconv = true; L.xcd_remap = 1;
+    return launch_transform(L, (hipStream_t)stream);
+}
+
+int ntt_launch_intt_plain(uint32_t* A_dev, const uint32_t* inv_twiddle_dev, const uint32_t* moduli_dev,
+                          const uint32_t* degree_inv_dev, size_t limb_count, int logN,
+                          const ntt_transform_config* cfg, void* stream) {
+    TransformLaunch L{};
+    int rc = resolve_cfg(logN, cfg, &L.cfg);
+    if (rc) return rc;
+    if (!degree_inv_dev) return NTT_ERR_INVALID_ARGUMENT;
+    L.A = A_dev; L.tw = inv_twiddle_dev; L.moduli = moduli_dev; L.q_inv = nullptr;
+    L.scale = degree_inv_dev; L.scale_mode = SCALE_TO_MONT;   // x * N^-1 (v03:516-527)
+    L.logN = logN; L.limbs = limb_count; L.count = 1; L.inverse = true; L.twconv = true; L.xcd_remap = 1;
+    return launch_transform(L, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,33 @@
+// Internal launch description shared by the launchers, plans and batcher.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/ntt_amd.h"
+
+namespace ntt {
+
+struct TransformLaunch {
+    uint32_t* A;               // operand(s), A[(op * limbs + l) * N + i]
+    const uint32_t* tw;        // Montgomery-form (or, with twconv, normal-domain) table per limb
+    const uint32_t* const* tw_ptrs;  // device array of per-limb table pointers (overrides tw)
+    const uint32_t* moduli;    // q per limb
+    const uint32_t* q_inv;     // -q^-1 mod 2^32 per limb, or null
+    const uint32_t* scale;     // per-limb scale source (see ScaleMode)
+    int scale_mode;
+    int logN;
+    size_t limbs;
+    size_t count;              // operands laid out back to back
+    bool inverse;
+    bool twconv;               // table is normal domain: convert each twiddle on load
+    int xcd_remap;
+    ntt_transform_config cfg;
+};
+
+int launch_transform(const TransformLaunch& L, hipStream_t stream);
+size_t device_max_lds();
+
+}  // namespace ntt

@@ -1,0 +1,213 @@
+// Contexts (device-resident twiddle cache) and plans of the ntt_amd C ABI.
+//
+// The reference builds every table on the host and uploads it inside main()
+// for one fixed modulus set (src/v04_montgomery.cu:882-916, 947-967).  Here a
+// context owns one Montgomery-form table per (q, logN, direction), shared by
+// every plan and batch that uses that modulus, and kept in HBM until the
+// context is destroyed.  A plan binds a modulus list to device arrays of
+// per-limb constants and table pointers.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ntt_amd.h"
+#include "context.hpp"
+#include "host_math.hpp"
+#include "launch.hpp"
+#include "phase_kernels.hpp"
+
+namespace ntt {
+
+int TwiddleCache::get(uint32_t q, uint32_t psi_root, int logN, bool inverse, const uint32_t** out) {
+    std::lock_guard<std::mutex> g(mu);
+    auto key = std::make_tuple(q, logN, inverse ? 1 : 0);
+    auto it = tables.find(key);
+    if (it != tables.end()) {
+        *out = it->second;
+        return NTT_OK;
+    }
+    const size_t N = (size_t)1 << logN;
+    std::vector<uint32_t> host(N);
+    twiddles_bitrev(logN, q, psi_root, true, host.data());
+    uint32_t* d = nullptr;
+    if (hipMalloc(&d, N * sizeof(uint32_t)) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
+    if (hipMemcpy(d, host.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return NTT_ERR_HIP;
+    }
+    tables[key] = d;
+    bytes += N * sizeof(uint32_t);
+    *out = d;
+    return NTT_OK;
+}
+
+TwiddleCache::~TwiddleCache() {
+    for (auto& kv : tables) (void)hipFree(kv.second);
+}
+
+int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs, int domain,
+               const ntt_transform_config* cfg, ntt_plan* P) {
+    P->ctx = ctx;
+    P->logN = logN;
+    P->limbs = limbs;
+    P->domain = domain;
+    if (cfg) P->cfg = *cfg;
+    else if (!ctx->tuned_config(logN, limbs, &P->cfg)) default_config(logN, &P->cfg);
+    int rc = validate_config(logN, &P->cfg, device_max_lds());
+    if (rc) return rc;
+
+    std::vector<uint32_t> q(limbs), qi(limbs), sf(limbs), si(limbs);
+    std::vector<const uint32_t*> twf(limbs), twi(limbs);
+    for (size_t l = 0; l < limbs; ++l) {
+        ntt_modulus_runtime rt;
+        if ((rc = make_runtime(logN, moduli[l].qi, moduli[l].primitive_root, &rt))) return rc;
+        q[l] = rt.qi;
+        qi[l] = rt.q_inv;
+        // Forward input scale: the Montgomery variant multiplies by R^2 (-> x R);
+        // the plain variant skips the multiply (see plan_launch).
+        sf[l] = rt.r2;
+        // Inverse output scale: mont_mul(v, K) with K = N^-1 for the
+        // Montgomery variant (drops the R factor too) and to_mont(N^-1) for plain.
+        si[l] = domain == NTT_DOMAIN_MONTGOMERY ? rt.degree_inv : to_mont_host(rt.degree_inv, rt.qi);
+        if ((rc = ctx->twiddles.get(rt.qi, rt.psi, logN, false, &twf[l]))) return rc;
+        if ((rc = ctx->twiddles.get(rt.qi, rt.psi_inv, logN, true, &twi[l]))) return rc;
+    }
+    const size_t u32b = limbs * sizeof(uint32_t), ptrb = limbs * sizeof(void*);
+    if (hipMalloc(&P->d_const, 4 * u32b + 2 * ptrb) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
+    char* base = reinterpret_cast<char*>(P->d_const);
+    P->d_tw_fwd = reinterpret_cast<const uint32_t* const*>(base);
+    P->d_tw_inv = reinterpret_cast<const uint32_t* const*>(base + ptrb);
+    P->d_moduli = reinterpret_cast<uint32_t*>(base + 2 * ptrb);
+    P->d_q_inv = P->d_moduli + limbs;
+    P->d_scale_fwd = P->d_q_inv + limbs;
+    P->d_scale_inv = P->d_scale_fwd + limbs;
+    std::vector<char> h(4 * u32b + 2 * ptrb);
+    std::memcpy(h.data(), twf.data(), ptrb);
+    std::memcpy(h.data() + ptrb, twi.data(), ptrb);
+    std::memcpy(h.data() + 2 * ptrb, q.data(), u32b);
+    std::memcpy(h.data() + 2 * ptrb + u32b, qi.data(), u32b);
+    std::memcpy(h.data() + 2 * ptrb + 2 * u32b, sf.data(), u32b);
+    std::memcpy(h.data() + 2 * ptrb + 3 * u32b, si.data(), u32b);
+    if (hipMemcpy(P->d_const, h.data(), h.size(), hipMemcpyHostToDevice) != hipSuccess) return NTT_ERR_HIP;
+    return NTT_OK;
+}
+
+TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool inverse) {
+    TransformLaunch L{};
+    L.A = A;
+    L.tw = nullptr;
+    L.tw_ptrs = inverse ? P->d_tw_inv : P->d_tw_fwd;
+    L.moduli = P->d_moduli;
+    L.q_inv = P->d_q_inv;
+    L.scale = inverse ? P->d_scale_inv : P->d_scale_fwd;
+    // The plain forward transform needs no input multiply.
+    L.scale_mode = (!inverse && P->domain == NTT_DOMAIN_PLAIN) ? SCALE_NONE : SCALE_AS_IS;
+    L.logN = P->logN;
+    L.limbs = P->limbs;
+    L.count = count;
+    L.inverse = inverse;
+    L.twconv = false;
+    L.xcd_remap = 1;
+    L.cfg = P->cfg;
+    return L;
+}
+
+}  // namespace ntt
+
+using namespace ntt;
+
+bool ntt_context::tuned_config(int logN, size_t limbs, ntt_transform_config* out) const {
+    std::lock_guard<std::mutex> g(tune_mu);
+    auto it = tuned.find(std::make_pair(logN, limbs));
+    if (it == tuned.end()) {
+        // nearest limb bucket at the same logN
+        const std::pair<int, size_t>* best = nullptr;
+        size_t bestd = ~(size_t)0;
+        for (auto& kv : tuned) {
+            if (kv.first.first != logN) continue;
+            size_t d = kv.first.second > limbs ? kv.first.second - limbs : limbs - kv.first.second;
+            if (d < bestd) { bestd = d; best = &kv.first; }
+        }
+        if (!best) return false;
+        *out = tuned.at(*best);
+        return true;
+    }
+    *out = it->second;
+    return true;
+}
+
+extern "C" {
+
+int ntt_context_create(int device, ntt_context** out) {
+    if (!out) return NTT_ERR_INVALID_ARGUMENT;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return NTT_ERR_NO_DEVICE;
+    if (device < 0 || device >= n) return NTT_ERR_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return NTT_ERR_HIP;
+    ntt_context* c = new ntt_context();
+    c->device = device;
+    *out = c;
+    return NTT_OK;
+}
+
+int ntt_context_destroy(ntt_context* ctx) {
+    if (!ctx) return NTT_ERR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    delete ctx;
+    return NTT_OK;
+}
+
+size_t ntt_context_twiddle_bytes(const ntt_context* ctx) { return ctx ? ctx->twiddles.bytes : 0; }
+size_t ntt_context_twiddle_entries(const ntt_context* ctx) { return ctx ? ctx->twiddles.tables.size() : 0; }
+
+int ntt_plan_create(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs, int domain,
+                    const ntt_transform_config* cfg, ntt_plan** out) {
+    if (!ctx || !moduli || !out || limbs == 0 || limbs > 0xffffffu) return NTT_ERR_INVALID_ARGUMENT;
+    if (logN < 1 || logN > NTT_MAX_LOG_N) return NTT_ERR_INVALID_ARGUMENT;
+    if (domain != NTT_DOMAIN_PLAIN && domain != NTT_DOMAIN_MONTGOMERY) return NTT_ERR_INVALID_ARGUMENT;
+    ntt_plan* P = new ntt_plan();
+    int rc = plan_build(ctx, logN, moduli, limbs, domain, cfg, P);
+    if (rc) {
+        ntt_plan_destroy(P);
+        return rc;
+    }
+    *out = P;
+    return NTT_OK;
+}
+
+int ntt_plan_destroy(ntt_plan* plan) {
+    if (!plan) return NTT_ERR_INVALID_ARGUMENT;
+    if (plan->d_const) (void)hipFree(plan->d_const);
+    delete plan;
+    return NTT_OK;
+}
+
+int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out) {
+    if (!plan || !out) return NTT_ERR_INVALID_ARGUMENT;
+    *out = plan->cfg;
+    return NTT_OK;
+}
+
+int ntt_plan_forward_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream) {
+    if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
+    return launch_transform(plan_launch(plan, A_dev, count, false), (hipStream_t)stream);
+}
+
+int ntt_plan_inverse_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream) {
+    if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
+    return launch_transform(plan_launch(plan, A_dev, count, true), (hipStream_t)stream);
+}
+
+int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream) {
+    return ntt_plan_forward_many(plan, A_dev, 1, stream);
+}
+
+int ntt_plan_inverse(ntt_plan* plan, uint32_t* A_dev, void* stream) {
+    return ntt_plan_inverse_many(plan, A_dev, 1, stream);
+}
+
+}  // extern "C"
