@@ -85,6 +85,8 @@ typedef struct {
  * Library / device information
  * ------------------------------------------------------------------------- */
 const char* ntt_status_string(int status);
+/* Text of the last HIP error this thread's calls turned into NTT_ERR_HIP. */
+const char* ntt_last_hip_error(void);
 const char* ntt_version(void);
 /* Max logN supported by the kernels (17 for the default phase split, up to 20). */
 int ntt_max_log_n(void);

@@ -10,9 +10,10 @@ Python mirror of the reference's host API over the library's C ABI
   launch_ntt_plain / launch_intt_plain          src/v03_template_config.cu:549-603
   make_modulus_runtime, make_twiddle_table      src/v04_montgomery.cu:681-727
 
-Device buffers are passed as torch tensors (any 32-bit dtype; the bits are
-read as uint32) or as raw integer device addresses.  The HIP library is
-required: importing this package raises if libntt_amd.so is missing.
+Device buffers are ntt_amd.device.DeviceBuffer objects (allocated through the
+library's own HIP runtime, include/ntt_amd_runtime.h) or raw integer device
+addresses.  The HIP library is required: importing this package raises if
+libntt_amd.so is missing.
 """
 from __future__ import annotations
 
@@ -30,7 +31,7 @@ HEADER = os.path.join(os.path.dirname(_HERE), "include", "ntt_amd.h")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
-        f"ntt_amd: HIP library not built ({LIB_PATH} missing); run `python -m ntt_amd.build`")
+        f"ntt_amd: HIP library not built ({LIB_PATH} missing); run `python ntt_amd/build.py`")
 
 _lib = ctypes.CDLL(LIB_PATH)
 
@@ -49,6 +50,8 @@ class NTTError(RuntimeError):
     def __init__(self, status: int, what: str = ""):
         self.status = status
         msg = _lib.ntt_status_string(status).decode()
+        if status == 6:
+            msg += " — " + (_lib.ntt_last_hip_error() or b"").decode()
         super().__init__(f"{what}: {STATUS.get(status, status)} ({msg})" if what else msg)
 
 
@@ -123,6 +126,7 @@ _PCFG = ctypes.POINTER(TransformConfig)
 
 _PROTOS = {
     "ntt_status_string": (ctypes.c_char_p, [_I]),
+    "ntt_last_hip_error": (ctypes.c_char_p, []),
     "ntt_version": (ctypes.c_char_p, []),
     "ntt_max_log_n": (_I, []),
     "ntt_default_moduli": (_I, [ctypes.POINTER(ModulusConfig), _I]),
@@ -165,6 +169,9 @@ def _bind(lib):
 
 _bind(_lib)
 
+from . import device  # noqa: E402  (binds include/ntt_amd_runtime.h)
+from .device import DeviceBuffer, Event, Stream, synchronize  # noqa: E402,F401
+
 
 def lib():
     return _lib
@@ -188,26 +195,23 @@ def _check(rc: int, what: str):
 
 
 def _addr(x) -> Optional[int]:
-    """Device address of a torch tensor, or an int address, or None."""
+    """Device address of a DeviceBuffer, an object with data_ptr(), or an int."""
     if x is None:
         return None
     if isinstance(x, int):
         return x
+    if hasattr(x, "ptr"):
+        return int(x.ptr)
     return int(x.data_ptr())
 
 
 def _stream(stream) -> Optional[int]:
+    """hipStream_t as an int (None / 0 = the null stream)."""
     if stream is None:
-        try:
-            import torch
-            if torch.cuda.is_available():
-                return int(torch.cuda.current_stream().cuda_stream) or None
-        except Exception:
-            return None
         return None
     if isinstance(stream, int):
         return stream or None
-    return int(stream.cuda_stream) or None
+    return int(stream.handle) or None
 
 
 def _cfg_ptr(cfg: Optional[TransformConfig]):

@@ -2,7 +2,8 @@
 
 Compiles every source under ntt_amd/csrc into ntt_amd/libntt_amd.so with one
 hipcc invocation per translation unit (objects under ntt_amd/_build/), then
-links.  Run as `python -m ntt_amd.build` or through __graft_entry__.build().
+links.  Run as `python ntt_amd/build.py` (not as a package module: importing
+the package loads the library) or through __graft_entry__.build().
 """
 from __future__ import annotations
 

@@ -42,6 +42,13 @@ static KernelFn pick(int S, bool inv, bool col, bool twconv) {
     }
 }
 
+static thread_local char g_last_error[256];
+
+int record_hip_error(hipError_t e, const char* where) {
+    snprintf(g_last_error, sizeof g_last_error, "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
+    return NTT_ERR_HIP;
+}
+
 static std::once_flag g_lds_once;
 static size_t g_max_lds = 64 * 1024;
 
@@ -72,12 +79,15 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, bool twco
     KernelFn fn = pick(S, inv, col, twconv);
     if (!fn) return NTT_ERR_CONFIG;
     if (lds > 64 * 1024) {
-        if (hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)lds) != hipSuccess)
-            return NTT_ERR_CONFIG;
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return record_hip_error(e, "hipFuncSetAttribute");
     }
+    hipError_t stale = hipGetLastError();   // clear an error left by an unrelated earlier call
+    if (stale != hipSuccess) record_hip_error(stale, "error pending before launch (cleared)");
     hipLaunchKernelGGL(fn, dim3((unsigned)blocks), dim3(threads), lds, stream, p);
-    return hipGetLastError() == hipSuccess ? NTT_OK : NTT_ERR_HIP;
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? NTT_OK : record_hip_error(e, "phase kernel launch");
 }
 
 int launch_transform(const TransformLaunch& L, hipStream_t stream) {
@@ -133,6 +143,8 @@ static int resolve_cfg(int logN, const ntt_transform_config* cfg, ntt_transform_
 using namespace ntt;
 
 extern "C" {
+
+const char* ntt_last_hip_error(void) { return g_last_error; }
 
 int ntt_launch_ntt_montgomery(uint32_t* A_dev, const uint32_t* twiddle_dev, const uint32_t* moduli_dev,
                               const uint32_t* q_inv_dev, const uint32_t* r2_dev, size_t limb_count, int logN,

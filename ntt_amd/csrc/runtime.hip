@@ -1,0 +1,140 @@
+// Thin device-runtime helpers (include/ntt_amd_runtime.h) over the HIP
+// runtime this library links.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/ntt_amd.h"
+#include "../../include/ntt_amd_runtime.h"
+#include "launch.hpp"
+
+namespace {
+
+inline int ck(hipError_t e, const char* where) {
+    return e == hipSuccess ? NTT_OK : ntt::record_hip_error(e, where);
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// One word per thread, grid-stride; value = floor(hash * q / 2^64).
+__global__ void fill_uniform_kernel(uint32_t* dst, const uint32_t* moduli, uint32_t limbs, int logN,
+                                    size_t total, uint64_t seed) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t limb = (uint32_t)((i >> logN) % limbs);
+        const uint64_t h = splitmix64(seed ^ splitmix64(i));
+        dst[i] = (uint32_t)__umul64hi(h, (uint64_t)moduli[limb]);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ntt_rt_device_count(int* count) {
+    if (!count) return NTT_ERR_INVALID_ARGUMENT;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = e == hipSuccess ? n : 0;
+    return e == hipSuccess ? NTT_OK : ck(e, "hipGetDeviceCount");
+}
+
+int ntt_rt_set_device(int device) { return ck(hipSetDevice(device), "hipSetDevice"); }
+
+int ntt_rt_device_info(int device, ntt_device_info* out) {
+    if (!out) return NTT_ERR_INVALID_ARGUMENT;
+    hipDeviceProp_t p;
+    int rc = ck(hipGetDeviceProperties(&p, device), "hipGetDeviceProperties");
+    if (rc) return rc;
+    std::memset(out, 0, sizeof *out);
+    std::snprintf(out->name, sizeof out->name, "%s", p.name);
+    std::snprintf(out->arch, sizeof out->arch, "%s", p.gcnArchName);
+    out->compute_units = p.multiProcessorCount;
+    out->clock_khz = p.clockRate;
+    out->l2_bytes = p.l2CacheSize;
+    out->lds_per_block = p.sharedMemPerBlock;
+    out->total_global_mem = p.totalGlobalMem;
+    return NTT_OK;
+}
+
+int ntt_rt_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return NTT_ERR_INVALID_ARGUMENT;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e == hipErrorOutOfMemory) { (void)hipGetLastError(); return NTT_ERR_OUT_OF_MEMORY; }
+    return ck(e, "hipMalloc");
+}
+
+int ntt_rt_free(void* ptr) { return ck(hipFree(ptr), "hipFree"); }
+
+int ntt_rt_memcpy_htod(void* dst, const void* src, size_t bytes) {
+    return ck(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+int ntt_rt_memcpy_dtoh(void* dst, const void* src, size_t bytes) {
+    return ck(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+int ntt_rt_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream) {
+    return ck(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream), "hipMemcpyAsync D2D");
+}
+
+int ntt_rt_memset(void* ptr, int value, size_t bytes) { return ck(hipMemset(ptr, value, bytes), "hipMemset"); }
+
+int ntt_rt_synchronize(void) { return ck(hipDeviceSynchronize(), "hipDeviceSynchronize"); }
+
+int ntt_rt_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return NTT_ERR_INVALID_ARGUMENT;
+    return ck(hipMemGetInfo(free_bytes, total_bytes), "hipMemGetInfo");
+}
+
+int ntt_rt_stream_create(void** stream) {
+    if (!stream) return NTT_ERR_INVALID_ARGUMENT;
+    return ck(hipStreamCreateWithFlags(reinterpret_cast<hipStream_t*>(stream), hipStreamNonBlocking),
+              "hipStreamCreate");
+}
+
+int ntt_rt_stream_destroy(void* stream) { return ck(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy"); }
+
+int ntt_rt_stream_synchronize(void* stream) {
+    return ck(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+}
+
+int ntt_rt_event_create(void** event) {
+    if (!event) return NTT_ERR_INVALID_ARGUMENT;
+    return ck(hipEventCreate(reinterpret_cast<hipEvent_t*>(event)), "hipEventCreate");
+}
+
+int ntt_rt_event_destroy(void* event) { return ck(hipEventDestroy((hipEvent_t)event), "hipEventDestroy"); }
+
+int ntt_rt_event_record(void* event, void* stream) {
+    return ck(hipEventRecord((hipEvent_t)event, (hipStream_t)stream), "hipEventRecord");
+}
+
+int ntt_rt_event_synchronize(void* event) {
+    return ck(hipEventSynchronize((hipEvent_t)event), "hipEventSynchronize");
+}
+
+int ntt_rt_event_elapsed_ms(float* ms, void* start, void* stop) {
+    if (!ms) return NTT_ERR_INVALID_ARGUMENT;
+    return ck(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
+}
+
+int ntt_rt_fill_uniform(uint32_t* dst, const uint32_t* moduli_dev, size_t limbs, int logN, size_t count,
+                        uint64_t seed, void* stream) {
+    if (!dst || !moduli_dev || limbs == 0 || count == 0 || logN < 1 || logN > 30) return NTT_ERR_INVALID_ARGUMENT;
+    const size_t total = (limbs * count) << logN;
+    const unsigned threads = 256;
+    size_t blocks = (total + threads - 1) / threads;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_uniform_kernel, dim3((unsigned)blocks), dim3(threads), 0, (hipStream_t)stream, dst,
+                       moduli_dev, (uint32_t)limbs, logN, total, seed);
+    return ck(hipGetLastError(), "fill_uniform launch");
+}
+
+}  // extern "C"

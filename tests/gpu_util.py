@@ -1,24 +1,24 @@
-"""Device-memory plumbing for the GPU tests (torch tensors hold the bits)."""
+"""Device plumbing for the GPU tests, over the library's own HIP runtime."""
 import numpy as np
 
 
-def torch_mod():
-    import torch
-    return torch
+def gpu_available() -> bool:
+    import ntt_amd
+    return ntt_amd.device.is_available()
 
 
 def to_dev(a: np.ndarray):
-    torch = torch_mod()
-    a = np.ascontiguousarray(a, dtype=np.uint32)
-    return torch.from_numpy(a.view(np.int32).copy()).cuda()
+    import ntt_amd
+    return ntt_amd.DeviceBuffer.from_numpy(np.ascontiguousarray(a, dtype=np.uint32))
 
 
-def to_host(t) -> np.ndarray:
-    return t.cpu().numpy().view(np.uint32).copy()
+def to_host(buf) -> np.ndarray:
+    return buf.numpy(np.uint32)
 
 
 def sync():
-    torch_mod().cuda.synchronize()
+    import ntt_amd
+    ntt_amd.synchronize()
 
 
 def random_operand(logN: int, moduli, seed: int) -> np.ndarray:

@@ -20,11 +20,10 @@ M48 = [tuple(x) for x in FIX["M48"]]
 
 @pytest.fixture(scope="module")
 def env(oracle_mod):
-    import torch
-    if not torch.cuda.is_available():
-        pytest.skip("no GPU")
     import ntt_amd
     from tests import gpu_util
+    if not gpu_util.gpu_available():
+        pytest.skip("no GPU")
     return ntt_amd, gpu_util, oracle_mod
 
 
