@@ -1,0 +1,271 @@
+#!/usr/bin/env python3
+"""Benchmark of the RNS multi-limb negacyclic NTT/INTT on one MI355X.
+
+Workload: Montgomery variant, N = 2^16, 16 limbs (the 16 largest primes
+q < 2^30 with q = 1 mod 2^17), a batch of B operands (default 256 = 1 GiB,
+four times the 256 MB last-level cache) resident in HBM.  One step is a
+forward NTT then an INTT of every operand in the batch (the batched path: one
+launch per pass over the whole batch).  Operands are uniform residues
+generated on the device; after the timed region the batch must equal its
+initial contents (the roundtrip check).
+
+Prints ONE JSON line on stdout (diagnostics go to stderr):
+  value      total butterflies (forward + inverse) per second, in G/s
+  roofline   the slowest pass kernel: algorithmic bytes per launch
+             (8 B per word: read once, written once) / its mean duration,
+             measured with HIP events on the launch stream
+  cpu_baseline  the oracle's butterfly loops (src/v04_montgomery.cu:742-784
+             restated in C) on one host core, table generation excluded
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "NTT/INTT Gbutterfly/s & %HBM peak, logN=16×16 limbs; served p50/p99 ms"
+HBM_PEAK_GBS = 8000.0
+RTX4090_V04_NTT_GBFLY = 487.3     # reference README.md:209, 0.017 ms incl. restore copy
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--logN", type=int, default=16)
+    ap.add_argument("--limbs", type=int, default=16)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the per-transform and single-transform legs")
+    return ap.parse_args()
+
+
+def cpu_baseline(logN: int, moduli, seconds: float, threads: int = 1):
+    """Oracle butterfly loops on host cores over a bounded sample: forward +
+    inverse of whole limbs of the workload (seed 0 inputs), until `seconds`."""
+    import ctypes
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    L = orc.lib()
+    tabs = []
+    for q, g in moduli:
+        rc, rt = orc.make_modulus_runtime(logN, q, g)
+        tabs.append((q, orc.twiddle_table(logN, q, rt["psi"]), orc.twiddle_table(logN, q, rt["psi_inv"]),
+                     rt["degree_inv"]))
+    x = orc.fill_random_limbs(logN, moduli, 0)
+    P = ctypes.POINTER(ctypes.c_uint32)
+
+    def one_limb(i):
+        q, tw, itw, dinv = tabs[i % len(tabs)]
+        a = x[i % len(tabs)].copy()
+        L.oracle_cpu_ntt_butterfly(a.ctypes.data_as(P), logN, q, tw.ctypes.data_as(P))
+        L.oracle_cpu_intt_butterfly(a.ctypes.data_as(P), logN, q, itw.ctypes.data_as(P), dinv)
+        return a
+
+    limbs_done = 0
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        while time.perf_counter() - t0 < seconds:
+            list(ex.map(one_limb, range(limbs_done, limbs_done + threads)))
+            limbs_done += threads
+    dt = time.perf_counter() - t0
+    bfly = 2 * 0.5 * (1 << logN) * logN * limbs_done
+    return {"value": bfly / dt / 1e9, "unit": "Gbutterfly/s", "cores": threads, "kind": "port",
+            "sample": f"{limbs_done} limbs of N=2^{logN} (forward + inverse each, seed-0 inputs, "
+                      f"tables prebuilt), {dt:.1f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}). "
+              "Limbs and batch operands shard across GPUs with no exchange.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import ntt_amd
+    from ntt_amd import device as dev
+
+    if not dev.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    info = dev.device_info(0)
+    log(f"device {info['name']} {info['arch']} CUs {info['compute_units']}")
+
+    logN, limbs, B = args.logN, args.limbs, args.batch
+    N = 1 << logN
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    ctx = ntt_amd.Context(0)
+    plan = ctx.plan(logN, moduli, ntt_amd.MONTGOMERY)
+    cfg = plan.config
+    words_per_op = limbs * N
+    pool = dev.DeviceBuffer(4 * words_per_op * B)
+    qdev = dev.DeviceBuffer.from_numpy(np.array([m[0] for m in moduli], np.uint32))
+    dev.fill_uniform(pool, qdev, limbs, logN, B, seed=0)
+    stream = dev.Stream()
+    dev.synchronize()
+    probe = [0, B - 1]
+    before = {k: pool.numpy(np.uint32, offset_bytes=4 * words_per_op * k, nbytes=4 * words_per_op) for k in probe}
+
+    bfly_per_transform = ntt_amd.butterflies(logN, limbs)
+    bytes_per_transform = ntt_amd.algorithmic_bytes(logN, limbs)
+
+    def step():
+        plan.forward(pool, B, stream)
+        plan.inverse(pool, B, stream)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    e0, e1 = dev.Event(), dev.Event()
+    dev.synchronize()
+    e0.record(stream)
+    t_wall = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    e1.synchronize()
+    t_wall = time.perf_counter() - t_wall
+    dev.synchronize()
+    total_ms = e0.elapsed_ms(e1)
+    ms_per_step = total_ms / args.steps
+    value = 2 * B * bfly_per_transform / (ms_per_step * 1e-3) / 1e9
+    hbm_frac_transform = 2 * B * bytes_per_transform / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS
+    log(f"batched: {ms_per_step:.3f} ms/step ({B} x fwd+inv), {value:.1f} Gbfly/s, wall {t_wall:.3f} s")
+
+    ok = all(np.array_equal(before[k], pool.numpy(np.uint32, offset_bytes=4 * words_per_op * k,
+                                                   nbytes=4 * words_per_op)) for k in probe)
+    if not ok:
+        log("ROUNDTRIP MISMATCH after the timed region")
+
+    # ---- per-pass kernel durations (events on the launch stream) ----
+    names = ["ntt_pass0_column", "ntt_pass1_row", "intt_pass0_row", "intt_pass1_column"]
+    ev = [dev.Event() for _ in range(5)]
+    per = {n: [] for n in names}
+    for _ in range(max(5, args.steps // 2)):
+        ev[0].record(stream)
+        plan.run_pass(pool, B, 0, 0, stream); ev[1].record(stream)
+        plan.run_pass(pool, B, 0, 1, stream); ev[2].record(stream)
+        plan.run_pass(pool, B, 1, 0, stream); ev[3].record(stream)
+        plan.run_pass(pool, B, 1, 1, stream); ev[4].record(stream)
+        ev[4].synchronize()
+        for i, n in enumerate(names):
+            per[n].append(ev[i].elapsed_ms(ev[i + 1]))
+    pass_ms = {n: statistics.mean(v) for n, v in per.items()}
+    dom = max(pass_ms, key=pass_ms.get)
+    achieved = B * bytes_per_transform / (pass_ms[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "kernel_ms": round(pass_ms[dom], 4),
+                "bytes_per_launch": B * bytes_per_transform}
+    log("pass ms: " + ", ".join(f"{n} {v:.4f}" for n, v in pass_ms.items()))
+
+    extras = {}
+    if not args.no_extras:
+        # plain per-transform launch: one plan call per operand
+        ev_a, ev_b = dev.Event(), dev.Event()
+        reps = 3
+        ev_a.record(stream)
+        for _ in range(reps):
+            for k in range(B):
+                off = pool.ptr + 4 * words_per_op * k
+                plan.forward(off, 1, stream)
+                plan.inverse(off, 1, stream)
+        ev_b.record(stream)
+        ev_b.synchronize()
+        ms = ev_a.elapsed_ms(ev_b) / reps
+        extras["per_transform_launch_gbfly_s"] = round(2 * B * bfly_per_transform / (ms * 1e-3) / 1e9, 1)
+        # single transform, reference-style (restore D2D copy + transform, src/v04_montgomery.cu:980-988)
+        one = dev.DeviceBuffer(4 * words_per_op)
+        src = dev.DeviceBuffer(4 * words_per_op)
+        src.copy_from(pool, 4 * words_per_op, stream)
+        for _ in range(10):
+            one.copy_from(src, stream=stream)
+            plan.forward(one, 1, stream)
+        iters = 30
+        ev_a.record(stream)
+        for _ in range(iters):
+            one.copy_from(src, stream=stream)
+            plan.forward(one, 1, stream)
+        ev_b.record(stream)
+        ev_b.synchronize()
+        ref_ms = ev_a.elapsed_ms(ev_b) / iters
+        ev_a.record(stream)
+        for _ in range(iters):
+            plan.forward(one, 1, stream)
+        ev_b.record(stream)
+        ev_b.synchronize()
+        k_ms = ev_a.elapsed_ms(ev_b) / iters
+        ev_a.record(stream)
+        for _ in range(iters):
+            plan.inverse(one, 1, stream)
+        ev_b.record(stream)
+        ev_b.synchronize()
+        ki_ms = ev_a.elapsed_ms(ev_b) / iters
+        extras.update({
+            "single_ntt_ms_reference_style": round(ref_ms, 4),
+            "single_ntt_gbfly_s_reference_style": round(bfly_per_transform / (ref_ms * 1e-3) / 1e9, 1),
+            "single_ntt_ms_kernel_only": round(k_ms, 4),
+            "single_intt_ms_kernel_only": round(ki_ms, 4),
+            "vs_rtx4090_v04_ntt_reference_style": round(bfly_per_transform / (ref_ms * 1e-3) / 1e9
+                                                        / RTX4090_V04_NTT_GBFLY, 3),
+        })
+        log("extras: " + json.dumps(extras))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(logN, moduli, args.cpu_seconds, threads=1)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Gbutterfly/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"Montgomery NTT then INTT, N=2^{logN}, {limbs} limbs (largest primes < 2^30, "
+                        f"q = 1 mod 2^{logN + 1}), batch of {B} operands "
+                        f"({4 * words_per_op * B / 2**30:.2f} GiB, HBM-resident), batched launch",
+            "logN": logN, "limbs": limbs, "batch": B,
+            "phase_config": cfg.astuple(),
+            "parallelism": "single GPU",
+        },
+        "hbm_frac_transform": round(hbm_frac_transform, 4),
+        "roundtrip_ok": ok,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
+        **extras,
+    }
+    print(json.dumps(out), flush=True)
+    if not ok:
+        sys.exit(3)
+
+
+if __name__ == "__main__":
+    main()

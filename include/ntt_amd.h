@@ -187,6 +187,13 @@ int ntt_plan_inverse(ntt_plan* plan, uint32_t* A_dev, void* stream);
  * (operand k at A_dev + k * limbs * N) in one launch per phase. */
 int ntt_plan_forward_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream);
 int ntt_plan_inverse_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream);
+/* One of the two passes alone (pass 0 runs first: the column pass of a
+ * forward transform, the row pass of an inverse one); the two in order equal
+ * the whole transform.  For per-kernel timing, and for callers that put work
+ * between the passes (e.g. a transpose across devices).  An empty pass
+ * (radix_stages 0) is a no-op. */
+int ntt_plan_run_pass(ntt_plan* plan, uint32_t* A_dev, size_t count, int direction, int pass,
+                      void* stream);
 
 #ifdef __cplusplus
 }

@@ -157,6 +157,7 @@ _PROTOS = {
     "ntt_plan_inverse": (_I, [_P, _P, _P]),
     "ntt_plan_forward_many": (_I, [_P, _P, _SZ, _P]),
     "ntt_plan_inverse_many": (_I, [_P, _P, _SZ, _P]),
+    "ntt_plan_run_pass": (_I, [_P, _P, _SZ, _I, _I, _P]),
 }
 
 
@@ -393,6 +394,10 @@ class Plan:
 
     def inverse(self, A, count: int = 1, stream=None):
         _check(_lib.ntt_plan_inverse_many(self.handle, _addr(A), count, _stream(stream)), "ntt_plan_inverse")
+
+    def run_pass(self, A, count: int, direction: int, pass_index: int, stream=None):
+        _check(_lib.ntt_plan_run_pass(self.handle, _addr(A), count, direction, pass_index, _stream(stream)),
+               "ntt_plan_run_pass")
 
     def close(self):
         if self.handle:

@@ -90,7 +90,9 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, bool twco
     return e == hipSuccess ? NTT_OK : record_hip_error(e, "phase kernel launch");
 }
 
-int launch_transform(const TransformLaunch& L, hipStream_t stream) {
+int launch_transform(const TransformLaunch& L, hipStream_t stream) { return launch_transform_passes(L, stream, 3); }
+
+int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pass_mask) {
     if (!L.A || (!L.tw && !L.tw_ptrs) || !L.moduli || L.limbs == 0 || L.count == 0) return NTT_ERR_INVALID_ARGUMENT;
     if (L.logN < 1 || L.logN > NTT_MAX_LOG_N) return NTT_ERR_INVALID_ARGUMENT;
     if (L.scale_mode != SCALE_NONE && !L.scale) return NTT_ERR_INVALID_ARGUMENT;
@@ -121,13 +123,17 @@ int launch_transform(const TransformLaunch& L, hipStream_t stream) {
     if (!L.inverse) {
         const ntt_phase_config &c1 = L.cfg.ntt_phase1, &c2 = L.cfg.ntt_phase2;
         const bool first_is_col = c1.radix_stages > 0;
-        if ((rc = launch_pass(c1, false, true, L.twconv, first_is_col ? scaled : none, stream))) return rc;
-        if ((rc = launch_pass(c2, false, false, L.twconv, first_is_col ? none : scaled, stream))) return rc;
+        if ((pass_mask & 1) && (rc = launch_pass(c1, false, true, L.twconv, first_is_col ? scaled : none, stream)))
+            return rc;
+        if ((pass_mask & 2) && (rc = launch_pass(c2, false, false, L.twconv, first_is_col ? none : scaled, stream)))
+            return rc;
     } else {
         const ntt_phase_config &c1 = L.cfg.intt_phase1, &c2 = L.cfg.intt_phase2;
         const bool last_is_col = c2.radix_stages > 0;
-        if ((rc = launch_pass(c1, true, false, L.twconv, last_is_col ? none : scaled, stream))) return rc;
-        if ((rc = launch_pass(c2, true, true, L.twconv, last_is_col ? scaled : none, stream))) return rc;
+        if ((pass_mask & 1) && (rc = launch_pass(c1, true, false, L.twconv, last_is_col ? none : scaled, stream)))
+            return rc;
+        if ((pass_mask & 2) && (rc = launch_pass(c2, true, true, L.twconv, last_is_col ? scaled : none, stream)))
+            return rc;
     }
     return NTT_OK;
 }

@@ -28,6 +28,8 @@ struct TransformLaunch {
 };
 
 int launch_transform(const TransformLaunch& L, hipStream_t stream);
+// pass_mask bit 0: the first pass, bit 1: the second.
+int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pass_mask);
 // Records the HIP error text (ntt_last_hip_error) and returns NTT_ERR_HIP.
 int record_hip_error(hipError_t e, const char* where);
 size_t device_max_lds();

@@ -202,6 +202,13 @@ int ntt_plan_inverse_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* s
     return launch_transform(plan_launch(plan, A_dev, count, true), (hipStream_t)stream);
 }
 
+int ntt_plan_run_pass(ntt_plan* plan, uint32_t* A_dev, size_t count, int direction, int pass, void* stream) {
+    if (!plan || !A_dev || count == 0 || (pass != 0 && pass != 1)) return NTT_ERR_INVALID_ARGUMENT;
+    if (direction != NTT_FORWARD && direction != NTT_INVERSE) return NTT_ERR_INVALID_ARGUMENT;
+    return launch_transform_passes(plan_launch(plan, A_dev, count, direction == NTT_INVERSE), (hipStream_t)stream,
+                                   1 << pass);
+}
+
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream) {
     return ntt_plan_forward_many(plan, A_dev, 1, stream);
 }
