@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the per-transform and single-transform legs")
+    ap.add_argument("--no-tune", action="store_true", help="use the default phase config instead of the autotuner's")
     return ap.parse_args()
 
 
@@ -113,8 +114,15 @@ def main():
     N = 1 << logN
     moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
     ctx = ntt_amd.Context(0)
+    tuning = {}
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuning["batch"] = ctx.autotune(logN, moduli, count=B, reps=3)
+        tuning["single"] = ctx.autotune(logN, moduli, count=1, reps=5)
+        tuning["seconds"] = round(time.perf_counter() - t0, 2)
+        log(f"autotune: {json.dumps(tuning)}")
     plan = ctx.plan(logN, moduli, ntt_amd.MONTGOMERY)
-    cfg = plan.config
+    cfg = ntt_amd.TransformConfig.make(*tuning["batch"]["cfg"]) if tuning else plan.config
     words_per_op = limbs * N
     pool = dev.DeviceBuffer(4 * words_per_op * B)
     qdev = dev.DeviceBuffer.from_numpy(np.array([m[0] for m in moduli], np.uint32))
@@ -253,6 +261,7 @@ def main():
                         f"({4 * words_per_op * B / 2**30:.2f} GiB, HBM-resident), batched launch",
             "logN": logN, "limbs": limbs, "batch": B,
             "phase_config": cfg.astuple(),
+            "chunk_operands": tuning["batch"]["chunk"] if tuning else 0,
             "parallelism": "single GPU",
         },
         "hbm_frac_transform": round(hbm_frac_transform, 4),
@@ -260,6 +269,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
+        "autotune": tuning or None,
         **extras,
     }
     print(json.dumps(out), flush=True)

@@ -48,6 +48,13 @@ typedef enum {
 typedef enum { NTT_DOMAIN_PLAIN = 0, NTT_DOMAIN_MONTGOMERY = 1 } ntt_domain;
 typedef enum { NTT_FORWARD = 0, NTT_INVERSE = 1 } ntt_direction;
 
+/* Kernel family a plan launches on.  TWO_PASS: the reference's two-phase
+ * structure (column pass + row pass, one HBM round trip each).  FUSED: one
+ * 1024-thread workgroup per 2^16 words holds whole limbs in registers and LDS,
+ * one HBM round trip per transform (10 <= logN <= 16).  AUTO: the autotuned
+ * choice for the launch size, else FUSED where supported. */
+typedef enum { NTT_ENGINE_AUTO = 0, NTT_ENGINE_TWO_PASS = 1, NTT_ENGINE_FUSED = 2 } ntt_engine;
+
 /* ModulusConfig, src/ntt_config.h:6-9 */
 typedef struct {
     uint32_t qi;
@@ -181,6 +188,10 @@ int ntt_plan_create(ntt_context* ctx, int logN, const ntt_modulus_config* moduli
                     ntt_plan** out);
 int ntt_plan_destroy(ntt_plan* plan);
 int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out);
+/* Selects the kernel family (ntt_engine); NTT_ERR_CONFIG if unsupported for logN. */
+int ntt_plan_set_engine(ntt_plan* plan, int engine);
+/* Engine a launch of `count` operands would use (AUTO resolved). */
+int ntt_plan_get_engine(const ntt_plan* plan, size_t count);
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream);
 int ntt_plan_inverse(ntt_plan* plan, uint32_t* A_dev, void* stream);
 /* Same transform over `count` operands laid out back to back
@@ -194,6 +205,33 @@ int ntt_plan_inverse_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* s
  * (radix_stages 0) is a no-op. */
 int ntt_plan_run_pass(ntt_plan* plan, uint32_t* A_dev, size_t count, int direction, int pass,
                       void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Autotuner.  No counterpart in the reference, which fixes its phase split at
+ * compile time (DefaultNTTLaunchConfig<LogN>, src/ntt_config.h:62-73).
+ * Measures every valid (radix split, stage_merging, warp_batching) per pass
+ * and direction, then the operands-per-chunk of multi-operand launches, for
+ * `limbs` x `count` operands of size 2^logN; stores the winner in the context
+ * for launches of that size (plans created without an explicit config use
+ * it).  Allocates a scratch operand of count * limbs * N words.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    ntt_transform_config cfg;
+    size_t chunk;               /* operands per chunk, 0 = whole launch */
+    double fwd_ms, inv_ms;      /* tuned forward / inverse time of the launch */
+    double default_fwd_ms, default_inv_ms;   /* same with ntt_default_transform_config */
+    int candidates;             /* configurations timed */
+    int engine;                 /* ntt_engine chosen (TWO_PASS or FUSED) */
+    double fused_fwd_ms, fused_inv_ms;       /* fused engine times (0 if not applicable) */
+} ntt_tune_result;
+
+int ntt_autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs,
+                 size_t count, int domain, int reps, ntt_tune_result* out);
+int ntt_context_set_tuning(ntt_context* ctx, int logN, size_t total_limbs,
+                           const ntt_transform_config* cfg, size_t chunk, int engine);
+int ntt_context_clear_tuning(ntt_context* ctx);
+int ntt_context_save_tuning(const ntt_context* ctx, const char* path);
+int ntt_context_load_tuning(ntt_context* ctx, const char* path);
 
 #ifdef __cplusplus
 }

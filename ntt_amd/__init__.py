@@ -44,6 +44,7 @@ STATUS = {
 }
 PLAIN, MONTGOMERY = 0, 1
 FORWARD, INVERSE = 0, 1
+ENGINE_AUTO, ENGINE_TWO_PASS, ENGINE_FUSED = 0, 1, 2
 
 
 class NTTError(RuntimeError):
@@ -108,6 +109,20 @@ class ModulusRuntime(ctypes.Structure):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
+class TuneResult(ctypes.Structure):
+    """ntt_tune_result (include/ntt_amd.h)."""
+    _fields_ = [("cfg", TransformConfig), ("chunk", ctypes.c_size_t), ("fwd_ms", ctypes.c_double),
+                ("inv_ms", ctypes.c_double), ("default_fwd_ms", ctypes.c_double),
+                ("default_inv_ms", ctypes.c_double), ("candidates", ctypes.c_int), ("engine", ctypes.c_int),
+                ("fused_fwd_ms", ctypes.c_double), ("fused_inv_ms", ctypes.c_double)]
+
+    def as_dict(self):
+        return {"cfg": self.cfg.astuple(), "chunk": int(self.chunk), "fwd_ms": self.fwd_ms, "inv_ms": self.inv_ms,
+                "default_fwd_ms": self.default_fwd_ms, "default_inv_ms": self.default_inv_ms,
+                "candidates": self.candidates, "engine": {1: "two_pass", 2: "fused"}.get(self.engine, self.engine),
+                "fused_fwd_ms": self.fused_fwd_ms, "fused_inv_ms": self.fused_inv_ms}
+
+
 @dataclass
 class VersionConfig:
     """VersionConfig{logN, moduli, transform}, src/ntt_config.h:25-29."""
@@ -158,6 +173,13 @@ _PROTOS = {
     "ntt_plan_forward_many": (_I, [_P, _P, _SZ, _P]),
     "ntt_plan_inverse_many": (_I, [_P, _P, _SZ, _P]),
     "ntt_plan_run_pass": (_I, [_P, _P, _SZ, _I, _I, _P]),
+    "ntt_plan_set_engine": (_I, [_P, _I]),
+    "ntt_plan_get_engine": (_I, [_P, _SZ]),
+    "ntt_autotune": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _SZ, _I, _I, ctypes.POINTER(TuneResult)]),
+    "ntt_context_set_tuning": (_I, [_P, _I, _SZ, _PCFG, _SZ, _I]),
+    "ntt_context_clear_tuning": (_I, [_P]),
+    "ntt_context_save_tuning": (_I, [_P, ctypes.c_char_p]),
+    "ntt_context_load_tuning": (_I, [_P, ctypes.c_char_p]),
 }
 
 
@@ -371,6 +393,30 @@ class Context:
     def plan(self, logN: int, moduli, domain: int = MONTGOMERY, cfg: Optional[TransformConfig] = None) -> "Plan":
         return Plan(self, logN, moduli, domain, cfg)
 
+    def autotune(self, logN: int, moduli, count: int = 1, domain: int = MONTGOMERY, reps: int = 3) -> dict:
+        """Measure phase configs for `count` operands of these moduli; the winner
+        is used by this context's plans for launches of that size."""
+        ms = as_moduli(moduli)
+        arr = (ModulusConfig * len(ms))(*ms)
+        r = TuneResult()
+        _check(_lib.ntt_autotune(self.handle, logN, arr, len(ms), count, domain, reps, ctypes.byref(r)),
+               "ntt_autotune")
+        return r.as_dict()
+
+    def set_tuning(self, logN: int, total_limbs: int, cfg: TransformConfig, chunk: int = 0,
+                   engine: int = ENGINE_TWO_PASS):
+        _check(_lib.ntt_context_set_tuning(self.handle, logN, total_limbs, ctypes.byref(cfg), chunk, engine),
+               "ntt_context_set_tuning")
+
+    def clear_tuning(self):
+        _check(_lib.ntt_context_clear_tuning(self.handle), "ntt_context_clear_tuning")
+
+    def save_tuning(self, path: str):
+        _check(_lib.ntt_context_save_tuning(self.handle, path.encode()), "ntt_context_save_tuning")
+
+    def load_tuning(self, path: str):
+        _check(_lib.ntt_context_load_tuning(self.handle, path.encode()), "ntt_context_load_tuning")
+
 
 class Plan:
     def __init__(self, ctx: Context, logN: int, moduli, domain: int = MONTGOMERY,
@@ -394,6 +440,12 @@ class Plan:
 
     def inverse(self, A, count: int = 1, stream=None):
         _check(_lib.ntt_plan_inverse_many(self.handle, _addr(A), count, _stream(stream)), "ntt_plan_inverse")
+
+    def set_engine(self, engine: int):
+        _check(_lib.ntt_plan_set_engine(self.handle, engine), "ntt_plan_set_engine")
+
+    def engine(self, count: int = 1) -> int:
+        return int(_lib.ntt_plan_get_engine(self.handle, count))
 
     def run_pass(self, A, count: int, direction: int, pass_index: int, stream=None):
         _check(_lib.ntt_plan_run_pass(self.handle, _addr(A), count, direction, pass_index, _stream(stream)),

@@ -23,14 +23,25 @@ struct TwiddleCache {
     ~TwiddleCache();
 };
 
+struct LimbConst;
+
+// Autotuner result for one (logN, words per launch) bucket.
+struct TunedEntry {
+    ntt_transform_config cfg;
+    int engine = NTT_ENGINE_TWO_PASS;   // NTT_ENGINE_TWO_PASS or NTT_ENGINE_FUSED
+    size_t chunk = 0;          // operands per chunk, 0 = whole launch
+    double fwd_ms = 0, inv_ms = 0;
+};
+
 }  // namespace ntt
 
 struct ntt_context {
     int device = 0;
     ntt::TwiddleCache twiddles;
     mutable std::mutex tune_mu;
-    std::map<std::pair<int, size_t>, ntt_transform_config> tuned;   // (logN, limbs) -> config
-    bool tuned_config(int logN, size_t limbs, ntt_transform_config* out) const;
+    // (logN, log2 of total limbs per launch) -> tuned entry
+    std::map<std::pair<int, int>, ntt::TunedEntry> tuned;
+    bool tuned_lookup(int logN, size_t total_limbs, ntt::TunedEntry* out) const;
 };
 
 struct ntt_plan {
@@ -38,7 +49,10 @@ struct ntt_plan {
     int logN = 0;
     size_t limbs = 0;
     int domain = NTT_DOMAIN_MONTGOMERY;
+    bool cfg_fixed = false;                        // config given by the caller
     ntt_transform_config cfg{};
+    int engine = NTT_ENGINE_AUTO;
+    size_t chunk = 0;
     void* d_const = nullptr;                       // one allocation holding the arrays below
     const uint32_t* const* d_tw_fwd = nullptr;     // per-limb forward table pointers
     const uint32_t* const* d_tw_inv = nullptr;
@@ -46,10 +60,21 @@ struct ntt_plan {
     uint32_t* d_q_inv = nullptr;
     uint32_t* d_scale_fwd = nullptr;               // Montgomery multiplier on the forward input
     uint32_t* d_scale_inv = nullptr;               // Montgomery multiplier on the inverse output
+    const ntt::LimbConst* d_lc_fwd = nullptr;      // per-limb constants for the fused kernel
+    const ntt::LimbConst* d_lc_inv = nullptr;
 };
 
 namespace ntt {
 int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs, int domain,
                const ntt_transform_config* cfg, ntt_plan* P);
+// The launch description of `count` operands; unless the caller fixed the
+// config, the tuned entry for this launch size (if any) supplies config and
+// chunking.
 TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool inverse);
+int log2_bucket(size_t v);
+// Engine a launch of `count` operands runs on (resolves NTT_ENGINE_AUTO).
+int plan_engine(const ntt_plan* P, size_t count);
+int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream);
+bool fused_supported(int logN);
+int device_cus();
 }  // namespace ntt

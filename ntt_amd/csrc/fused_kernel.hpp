@@ -1,0 +1,281 @@
+// Fused limb-per-workgroup negacyclic NTT / INTT for gfx950.
+//
+// One 1024-thread workgroup (16 wave64s, one per CU: it takes 132 KiB of the
+// CU's 160 KiB LDS) holds 64K words in registers: 2^(16-n) limbs of N = 2^n
+// words, 2^(n-6) threads x 64 words per limb.  Every stage of the transform
+// runs from registers; between rounds the words move through LDS in two
+// halves.  Each limb is read from and written to HBM exactly once, instead of
+// once per pass as in the two-pass kernels (src/v04_montgomery.cu:196-565 is
+// the reference's two-pass form).
+//
+// Schedule (all compile-time for a given n; checked by the host model in
+// tests/fused_model.py):
+//  * round k owns a 6-bit window [lo_k, lo_k + 6) of the word index; thread
+//    u of a limb holds the 64 words
+//        i = ((u >> lo) << (lo + 6)) | (x << lo) | (u & (2^lo - 1)),  x < 64;
+//  * forward windows go from the top bits down, inverse from the bottom up,
+//    and consecutive windows share a bit sigma; the exchange between them
+//    moves the words with sigma = 0, then those with sigma = 1, through a
+//    32K-word LDS buffer (word i of a limb at compress(i without sigma) plus
+//    one padding word per 32);
+//  * round k stores window bit b in register bit perm_k(b), a transposition
+//    chosen so that sigma occupies the same register bit on both sides of an
+//    exchange: in each half a thread reads back exactly the registers it has
+//    just written out;
+//  * the twiddle of the stage on window bit j of word x is
+//    tw[((2^(n-6-lo) + (u >> lo)) << (5-j)) + (x >> (j+1))].
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+#include <utility>
+
+#include "arith.hpp"
+#include "fused_types.hpp"
+
+namespace ntt {
+
+namespace fused {
+
+constexpr int EB = 6;
+constexpr int E = 64;
+constexpr int THREADS = 1024;
+constexpr uint32_t LDS_WORDS = 32768 + 1024;
+
+__host__ __device__ constexpr int nrounds(int n) { return n >= 12 ? 3 : 2; }
+
+__host__ __device__ constexpr int win_lo(int n, bool inv, int k) {
+    return !inv ? (n >= 13 ? (k == 0 ? n - 6 : k == 1 ? n - 11 : 0)
+                           : n == 12 ? (k == 0 ? 6 : k == 1 ? 1 : 0) : (k == 0 ? n - 6 : 0))
+                : (n >= 13 ? (k == 0 ? 0 : k == 1 ? 5 : n - 6)
+                           : n == 12 ? (k == 0 ? 0 : k == 1 ? 5 : 6) : (k == 0 ? 0 : n - 6));
+}
+
+// Window-relative range [jlo, jhi] of the stages round k performs.
+__host__ __device__ constexpr int stage_jhi(int n, bool inv, int k) {
+    return !inv ? (k == 0 ? 5 : win_lo(n, inv, k - 1) - 1 - win_lo(n, inv, k)) : 5;
+}
+__host__ __device__ constexpr int stage_jlo(int n, bool inv, int k) {
+    return !inv ? 0 : (k == 0 ? 0 : win_lo(n, inv, k - 1) + 6 - win_lo(n, inv, k));
+}
+
+__host__ __device__ constexpr int sigma(int n, bool inv, int k) {
+    return !inv ? win_lo(n, inv, k) : win_lo(n, inv, k) + 5;
+}
+
+// Register bit of window bit b in round k.
+__host__ __device__ constexpr int perm_bit(int n, bool inv, int k, int b) {
+    int p[EB] = {0, 1, 2, 3, 4, 5};
+    for (int r = 1; r <= k; ++r) {
+        const int s = sigma(n, inv, r - 1);
+        const int ra = p[s - win_lo(n, inv, r - 1)];
+        const int jb = s - win_lo(n, inv, r);
+        int q[EB] = {0, 1, 2, 3, 4, 5};
+        q[jb] = ra;
+        q[ra] = jb;
+        for (int i = 0; i < EB; ++i) p[i] = q[i];
+    }
+    return p[b];
+}
+
+__host__ __device__ constexpr int reg_of(int n, bool inv, int k, int x) {
+    int r = 0;
+    for (int b = 0; b < EB; ++b)
+        if ((x >> b) & 1) r |= 1 << perm_bit(n, inv, k, b);
+    return r;
+}
+
+// Register index as a constant expression.
+template <int n, bool INV, int K, int X>
+inline constexpr int REG = reg_of(n, INV, K, X);
+
+// Calls f(std::integral_constant<int, I>) for I = 0 .. N-1, so that every
+// register index derived from I is a constant expression.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+template <int LO>
+__device__ __forceinline__ uint32_t vidx(uint32_t u, int x) {
+    return ((u >> LO) << (LO + EB)) | ((uint32_t)x << LO) | (u & ((1u << LO) - 1));
+}
+
+template <int S>
+__device__ __forceinline__ uint32_t lds_index(uint32_t i) {
+    const uint32_t c = ((i >> (S + 1)) << S) | (i & ((1u << S) - 1));
+    return c + (c >> 5);
+}
+
+template <int n, bool INV, int K>
+__device__ __forceinline__ void stages(uint32_t (&v)[E], const uint32_t* __restrict__ tw, uint32_t u, uint32_t q,
+                                       uint32_t q_inv) {
+    constexpr int LO = win_lo(n, INV, K);
+    constexpr int JLO = stage_jlo(n, INV, K), JHI = stage_jhi(n, INV, K);
+    const uint32_t TB = (1u << (n - EB - LO)) + (u >> LO);
+    static_for<EB>([&](auto JJ) __attribute__((always_inline)) {
+        constexpr int j = INV ? decltype(JJ)::value : EB - 1 - decltype(JJ)::value;
+        if constexpr (j >= JLO && j <= JHI) {
+            static_for<E / 2>([&](auto P) __attribute__((always_inline)) {
+                // P enumerates the words with bit j clear: x = P with a 0 inserted at bit j
+                constexpr int p = decltype(P)::value;
+                constexpr int x = ((p >> j) << (j + 1)) | (p & ((1 << j) - 1));
+                constexpr int a = REG<n, INV, K, x>, b = REG<n, INV, K, x | (1 << j)>;
+                const uint32_t w = tw[(TB << (EB - 1 - j)) + (x >> (j + 1))];
+                if (INV)
+                    gs_butterfly(v[a], v[b], w, q, q_inv);
+                else
+                    ct_butterfly(v[a], v[b], w, q, q_inv);
+            });
+        }
+    });
+}
+
+template <int n, bool INV, int K>
+__device__ __forceinline__ void exchange(uint32_t (&v)[E], uint32_t* L, uint32_t u, bool valid) {
+    constexpr int S = sigma(n, INV, K);
+    constexpr int LOA = win_lo(n, INV, K), LOB = win_lo(n, INV, K + 1);
+    constexpr int JA = S - LOA, JB = S - LOB;
+    static_for<2>([&](auto H) __attribute__((always_inline)) {
+        constexpr int h = decltype(H)::value;
+        if (valid) {
+            static_for<E>([&](auto X) __attribute__((always_inline)) {
+                constexpr int x = decltype(X)::value;
+                if constexpr (((x >> JA) & 1) == h) L[lds_index<S>(vidx<LOA>(u, x))] = v[REG<n, INV, K, x>];
+            });
+        }
+        __syncthreads();
+        if (valid) {
+            static_for<E>([&](auto X) __attribute__((always_inline)) {
+                constexpr int x = decltype(X)::value;
+                if constexpr (((x >> JB) & 1) == h) v[REG<n, INV, K + 1, x>] = L[lds_index<S>(vidx<LOB>(u, x))];
+            });
+        }
+        __syncthreads();
+    });
+}
+
+template <int n, bool INV>
+__device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds) {
+    constexpr int G_BITS = n - EB;
+    constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 6));
+    constexpr int NR = nrounds(n);
+    constexpr int LO0 = win_lo(n, INV, 0), LOL = win_lo(n, INV, NR - 1);
+    const uint32_t t = threadIdx.x;
+    uint32_t slot = t >> G_BITS;
+    uint32_t u = t & ((1u << G_BITS) - 1);
+    // Opaque per item: otherwise the persistent loop lets the compiler hoist
+    // every address derived from u out of the loop and keep it live.
+    asm volatile("" : "+v"(u));
+    if (n >= 12) slot = __builtin_amdgcn_readfirstlane(slot);   // one limb per wave
+    const uint32_t limb = first_limb + slot;
+    const bool valid = limb < job.limbs;
+    const uint32_t lsafe = valid ? limb : first_limb;
+    const uint32_t per = lsafe % job.period;
+    uint32_t* A = job.data + ((size_t)lsafe << n);
+    const uint32_t* tw = job.tw[per];
+    const LimbConst c = job.lc[per];
+    uint32_t* L = lds + slot * STRIDE;
+
+    uint32_t v[E];
+    // ---- load (round 0 layout) ----
+    if (valid) {
+        if (LO0 == 0) {
+            const uint4* src = reinterpret_cast<const uint4*>(A + vidx<LO0>(u, 0));
+            static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
+                constexpr int x4 = decltype(X4)::value;
+                const uint4 w = src[x4];
+                v[REG<n, INV, 0, 4 * x4>] = w.x;
+                v[REG<n, INV, 0, 4 * x4 + 1>] = w.y;
+                v[REG<n, INV, 0, 4 * x4 + 2>] = w.z;
+                v[REG<n, INV, 0, 4 * x4 + 3>] = w.w;
+            });
+        } else {
+            static_for<E>([&](auto X) __attribute__((always_inline)) {
+                constexpr int x = decltype(X)::value;
+                v[REG<n, INV, 0, x>] = A[vidx<LO0>(u, x)];
+            });
+        }
+        if (!INV && (c.flags & 1)) {
+#pragma unroll
+            for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], c.scale, c.q, c.q_inv);
+        }
+    } else {
+#pragma unroll
+        for (int x = 0; x < E; ++x) v[x] = 0;
+    }
+
+    stages<n, INV, 0>(v, tw, u, c.q, c.q_inv);
+    exchange<n, INV, 0>(v, L, u, valid);
+    stages<n, INV, 1>(v, tw, u, c.q, c.q_inv);
+    if constexpr (NR == 3) {
+        exchange<n, INV, 1>(v, L, u, valid);
+        stages<n, INV, 2>(v, tw, u, c.q, c.q_inv);
+    }
+
+    // ---- store (last round layout) ----
+    if (valid) {
+        if (INV && (c.flags & 1)) {
+#pragma unroll
+            for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], c.scale, c.q, c.q_inv);
+        }
+        if (LOL == 0) {
+            uint4* dst = reinterpret_cast<uint4*>(A + vidx<LOL>(u, 0));
+            static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
+                constexpr int x4 = decltype(X4)::value;
+                dst[x4] = make_uint4(v[REG<n, INV, NR - 1, 4 * x4>], v[REG<n, INV, NR - 1, 4 * x4 + 1>],
+                                     v[REG<n, INV, NR - 1, 4 * x4 + 2>], v[REG<n, INV, NR - 1, 4 * x4 + 3>]);
+            });
+        } else {
+            static_for<E>([&](auto X) __attribute__((always_inline)) {
+                constexpr int x = decltype(X)::value;
+                A[vidx<LOL>(u, x)] = v[REG<n, INV, NR - 1, x>];
+            });
+        }
+    }
+}
+
+}  // namespace fused
+
+// Persistent loop over work items of jobs of one size 2^n and one direction;
+// an item is 2^(16-n) consecutive limbs of one job.  Every thread of the
+// workgroup runs every barrier of every item.
+template <int n, bool INV>
+__global__ __launch_bounds__(1024, 1) void fused_kernel(FusedParams p) {
+    extern __shared__ uint32_t lds[];
+    __shared__ uint32_t s_next;
+    uint32_t iter = 0;
+    for (;;) {
+        uint32_t item;
+        if (p.counter) {
+            if (threadIdx.x == 0) s_next = atomicAdd(p.counter, 1u);
+            __syncthreads();
+            item = s_next;
+            __syncthreads();
+        } else {
+            item = blockIdx.x + iter * gridDim.x;
+            ++iter;
+        }
+        item = __builtin_amdgcn_readfirstlane(item);
+        if (item >= p.total_items) break;
+        FusedJob job = p.job0;
+        if (p.jobs) {
+            // last job with item_begin <= item
+            uint32_t lo = 0, hi = p.njobs;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (p.jobs[mid].item_begin <= item) lo = mid; else hi = mid;
+            }
+            job = p.jobs[lo];
+        }
+        fused::run_item<n, INV>(job, (item - job.item_begin) << (16 - n), lds);
+    }
+}
+
+}  // namespace ntt

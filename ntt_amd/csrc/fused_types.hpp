@@ -1,0 +1,37 @@
+// Work descriptors of the fused limb kernel (fused_kernel.hpp), shared with
+// the host code that builds them.
+#pragma once
+
+#include <stdint.h>
+
+namespace ntt {
+
+struct LimbConst {
+    uint32_t q;
+    uint32_t q_inv;
+    uint32_t scale;    // Montgomery multiplier: forward input / inverse output
+    uint32_t flags;    // bit 0: apply scale
+};
+
+// A run of `limbs` limbs of size 2^n stored back to back at `data`; limb l
+// uses table tw[l % period] and constants lc[l % period].
+struct FusedJob {
+    uint32_t* data;
+    const uint32_t* const* tw;
+    const LimbConst* lc;
+    uint32_t limbs;
+    uint32_t period;
+    uint32_t item_begin;   // first work item of this job (prefix sum)
+    uint16_t n;
+    uint16_t inverse;
+};
+
+struct FusedParams {
+    FusedJob job0;            // used when jobs == nullptr
+    const FusedJob* jobs;     // device array, item_begin ascending
+    uint32_t njobs;
+    uint32_t total_items;
+    uint32_t* counter;        // dynamic scheduling; null = static round-robin
+};
+
+}  // namespace ntt

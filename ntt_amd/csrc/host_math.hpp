@@ -8,6 +8,9 @@
 
 #define NTT_MAX_LOG_N 24
 #define NTT_MAX_STAGE_MERGING 5
+// Sizes the fused limb kernel handles (one workgroup holds 2^16 words).
+#define FUSED_MIN_LOG_N 10
+#define FUSED_MAX_LOG_N 16
 // gfx950: 160 KiB of LDS per CU, all of it addressable by one workgroup.
 #define NTT_LDS_BYTES_GFX950 (160u * 1024u)
 

@@ -90,7 +90,18 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, bool twco
     return e == hipSuccess ? NTT_OK : record_hip_error(e, "phase kernel launch");
 }
 
-int launch_transform(const TransformLaunch& L, hipStream_t stream) { return launch_transform_passes(L, stream, 3); }
+int launch_transform(const TransformLaunch& L, hipStream_t stream) {
+    if (L.chunk == 0 || L.chunk >= L.count) return launch_transform_passes(L, stream, 3);
+    const size_t words = L.limbs << L.logN;
+    for (size_t first = 0; first < L.count; first += L.chunk) {
+        TransformLaunch c = L;
+        c.A = L.A + first * words;
+        c.count = L.count - first < L.chunk ? L.count - first : L.chunk;
+        int rc = launch_transform_passes(c, stream, 3);
+        if (rc) return rc;
+    }
+    return NTT_OK;
+}
 
 int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pass_mask) {
     if (!L.A || (!L.tw && !L.tw_ptrs) || !L.moduli || L.limbs == 0 || L.count == 0) return NTT_ERR_INVALID_ARGUMENT;

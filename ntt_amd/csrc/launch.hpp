@@ -24,6 +24,9 @@ struct TransformLaunch {
     bool inverse;
     bool twconv;               // table is normal domain: convert each twiddle on load
     int xcd_remap;
+    // Operands per chunk (0 = all): both passes run chunk by chunk, so the
+    // intermediate between the passes of one chunk can stay in the caches.
+    size_t chunk;
     ntt_transform_config cfg;
 };
 

@@ -16,6 +16,7 @@
 
 #include "../../include/ntt_amd.h"
 #include "context.hpp"
+#include "fused_types.hpp"
 #include "host_math.hpp"
 #include "launch.hpp"
 #include "phase_kernels.hpp"
@@ -55,8 +56,9 @@ int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, siz
     P->logN = logN;
     P->limbs = limbs;
     P->domain = domain;
+    P->cfg_fixed = cfg != nullptr;
     if (cfg) P->cfg = *cfg;
-    else if (!ctx->tuned_config(logN, limbs, &P->cfg)) default_config(logN, &P->cfg);
+    else default_config(logN, &P->cfg);
     int rc = validate_config(logN, &P->cfg, device_max_lds());
     if (rc) return rc;
 
@@ -77,7 +79,8 @@ int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, siz
         if ((rc = ctx->twiddles.get(rt.qi, rt.psi_inv, logN, true, &twi[l]))) return rc;
     }
     const size_t u32b = limbs * sizeof(uint32_t), ptrb = limbs * sizeof(void*);
-    if (hipMalloc(&P->d_const, 4 * u32b + 2 * ptrb) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
+    const size_t lcb = limbs * sizeof(LimbConst);
+    if (hipMalloc(&P->d_const, 2 * lcb + 4 * u32b + 2 * ptrb) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
     char* base = reinterpret_cast<char*>(P->d_const);
     P->d_tw_fwd = reinterpret_cast<const uint32_t* const*>(base);
     P->d_tw_inv = reinterpret_cast<const uint32_t* const*>(base + ptrb);
@@ -85,7 +88,16 @@ int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, siz
     P->d_q_inv = P->d_moduli + limbs;
     P->d_scale_fwd = P->d_q_inv + limbs;
     P->d_scale_inv = P->d_scale_fwd + limbs;
-    std::vector<char> h(4 * u32b + 2 * ptrb);
+    P->d_lc_fwd = reinterpret_cast<const LimbConst*>(base + 2 * ptrb + 4 * u32b);
+    P->d_lc_inv = P->d_lc_fwd + limbs;
+    std::vector<LimbConst> lcf(limbs), lci(limbs);
+    for (size_t l = 0; l < limbs; ++l) {
+        lcf[l] = {q[l], qi[l], sf[l], domain == NTT_DOMAIN_MONTGOMERY ? 1u : 0u};
+        lci[l] = {q[l], qi[l], si[l], 1u};
+    }
+    std::vector<char> h(2 * lcb + 4 * u32b + 2 * ptrb);
+    std::memcpy(h.data() + 2 * ptrb + 4 * u32b, lcf.data(), lcb);
+    std::memcpy(h.data() + 2 * ptrb + 4 * u32b + lcb, lci.data(), lcb);
     std::memcpy(h.data(), twf.data(), ptrb);
     std::memcpy(h.data() + ptrb, twi.data(), ptrb);
     std::memcpy(h.data() + 2 * ptrb, q.data(), u32b);
@@ -113,30 +125,45 @@ TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool i
     L.twconv = false;
     L.xcd_remap = 1;
     L.cfg = P->cfg;
+    L.chunk = P->chunk;
+    TunedEntry t;
+    if (!P->cfg_fixed && P->ctx->tuned_lookup(P->logN, P->limbs * count, &t)) {
+        L.cfg = t.cfg;
+        L.chunk = t.chunk;
+    }
     return L;
+}
+
+int plan_engine(const ntt_plan* P, size_t count) {
+    if (P->engine != NTT_ENGINE_AUTO) return P->engine;
+    TunedEntry t;
+    if (!P->cfg_fixed && P->ctx->tuned_lookup(P->logN, P->limbs * count, &t)) return t.engine;
+    if (P->cfg_fixed) return NTT_ENGINE_TWO_PASS;
+    return fused_supported(P->logN) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
+}
+
+int log2_bucket(size_t v) {
+    int b = 0;
+    while ((size_t(2) << b) <= v) ++b;
+    return b;
 }
 
 }  // namespace ntt
 
 using namespace ntt;
 
-bool ntt_context::tuned_config(int logN, size_t limbs, ntt_transform_config* out) const {
+bool ntt_context::tuned_lookup(int logN, size_t total_limbs, TunedEntry* out) const {
     std::lock_guard<std::mutex> g(tune_mu);
-    auto it = tuned.find(std::make_pair(logN, limbs));
-    if (it == tuned.end()) {
-        // nearest limb bucket at the same logN
-        const std::pair<int, size_t>* best = nullptr;
-        size_t bestd = ~(size_t)0;
-        for (auto& kv : tuned) {
-            if (kv.first.first != logN) continue;
-            size_t d = kv.first.second > limbs ? kv.first.second - limbs : limbs - kv.first.second;
-            if (d < bestd) { bestd = d; best = &kv.first; }
-        }
-        if (!best) return false;
-        *out = tuned.at(*best);
-        return true;
+    const int b = log2_bucket(total_limbs);
+    const TunedEntry* best = nullptr;
+    int bestd = 1 << 30;
+    for (auto& kv : tuned) {
+        if (kv.first.first != logN) continue;
+        const int d = kv.first.second > b ? kv.first.second - b : b - kv.first.second;
+        if (d < bestd) { bestd = d; best = &kv.second; }
     }
-    *out = it->second;
+    if (!best) return false;
+    *out = *best;
     return true;
 }
 
@@ -192,14 +219,33 @@ int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out) {
     return NTT_OK;
 }
 
-int ntt_plan_forward_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream) {
+static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse, void* stream) {
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
-    return launch_transform(plan_launch(plan, A_dev, count, false), (hipStream_t)stream);
+    if (plan_engine(plan, count) == NTT_ENGINE_FUSED)
+        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream);
+    return launch_transform(plan_launch(plan, A_dev, count, inverse), (hipStream_t)stream);
+}
+
+int ntt_plan_forward_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream) {
+    return plan_run(plan, A_dev, count, false, stream);
 }
 
 int ntt_plan_inverse_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream) {
-    if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
-    return launch_transform(plan_launch(plan, A_dev, count, true), (hipStream_t)stream);
+    return plan_run(plan, A_dev, count, true, stream);
+}
+
+int ntt_plan_set_engine(ntt_plan* plan, int engine) {
+    if (!plan) return NTT_ERR_INVALID_ARGUMENT;
+    if (engine != NTT_ENGINE_AUTO && engine != NTT_ENGINE_TWO_PASS && engine != NTT_ENGINE_FUSED)
+        return NTT_ERR_INVALID_ARGUMENT;
+    if (engine == NTT_ENGINE_FUSED && !fused_supported(plan->logN)) return NTT_ERR_CONFIG;
+    plan->engine = engine;
+    return NTT_OK;
+}
+
+int ntt_plan_get_engine(const ntt_plan* plan, size_t count) {
+    if (!plan || count == 0) return -NTT_ERR_INVALID_ARGUMENT;
+    return plan_engine(plan, count);
 }
 
 int ntt_plan_run_pass(ntt_plan* plan, uint32_t* A_dev, size_t count, int direction, int pass, void* stream) {

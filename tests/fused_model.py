@@ -1,0 +1,199 @@
+"""Host model of the fused limb kernel's schedule (ntt_amd/csrc/fused_kernel.hpp).
+
+A 1024-thread workgroup holds 64K words in registers: 2^(16-n) limbs of
+N = 2^n words, 2^(n-6) threads x 64 words per limb.  A round performs the
+pending stages of a 6-bit window of the word index in registers; between
+rounds the words move through LDS in two halves (LDS holds 32K words), split
+by a bit sigma that is a register bit of both windows.  Each round maps its
+window bits to register bits through a transposition chosen so sigma sits in
+the same register bit on both sides of an exchange: the registers a thread
+reads in half h are exactly those it wrote out in half h (no register is
+overwritten before it is saved).
+
+This model replays that schedule with the same formulas and checks bounds,
+the hazard rule above, LDS ownership, and the transform against the oracle.
+"""
+from __future__ import annotations
+
+E_BITS = 6
+E = 1 << E_BITS
+THREADS = 1024
+HALF_WORDS = 32768
+
+
+def windows(n: int, inv: bool):
+    """Window low bits of the rounds, in execution order."""
+    if not inv:
+        if n >= 13:
+            return [n - 6, n - 11, 0]
+        if n == 12:
+            return [6, 1, 0]
+        return [n - 6, 0]
+    if n >= 13:
+        return [0, 5, n - 6]
+    if n == 12:
+        return [0, 5, 6]
+    return [0, n - 6]
+
+
+def stage_bits(n: int, inv: bool):
+    """Per round, the window-relative bits whose stages run, in order."""
+    ws = windows(n, inv)
+    out = []
+    for k, lo in enumerate(ws):
+        if not inv:
+            top = lo + 5 if k == 0 else ws[k - 1] - 1
+            out.append([b - lo for b in range(top, lo - 1, -1)])
+        else:
+            bot = lo if k == 0 else ws[k - 1] + 6
+            out.append([b - lo for b in range(bot, lo + 6)])
+    return out
+
+
+def sigmas(n: int, inv: bool):
+    ws = windows(n, inv)
+    return [(ws[k] if not inv else ws[k] + 5) for k in range(len(ws) - 1)]
+
+
+def perms(n: int, inv: bool):
+    """perm[k][window bit] = register bit, per round."""
+    ws = windows(n, inv)
+    sg = sigmas(n, inv)
+    out = [list(range(E_BITS))]
+    for k in range(1, len(ws)):
+        s = sg[k - 1]
+        ra = out[k - 1][s - ws[k - 1]]
+        jb = s - ws[k]
+        assert 0 <= jb < E_BITS, "split bit must be a register bit of both windows"
+        p = list(range(E_BITS))
+        # transposition jb <-> ra
+        p[jb], p[ra] = ra, jb
+        out.append(p)
+    return out
+
+
+def reg_of(p, xw: int) -> int:
+    r = 0
+    for b in range(E_BITS):
+        if (xw >> b) & 1:
+            r |= 1 << p[b]
+    return r
+
+
+def vidx(lo: int, u: int, x: int) -> int:
+    return ((u >> lo) << (lo + E_BITS)) | (x << lo) | (u & ((1 << lo) - 1))
+
+
+def compress(i: int, s: int) -> int:
+    return ((i >> (s + 1)) << s) | (i & ((1 << s) - 1))
+
+
+def lds_slot_stride(n: int) -> int:
+    half = 1 << (n - 1)
+    return half + (half >> 5)
+
+
+def run(limbs_data, tws, qs, n: int, inv: bool, scale=None):
+    """limbs_data: list of 2^(16-n) limbs (lists), modified copies returned."""
+    N = 1 << n
+    g = n - E_BITS
+    lpw = 1 << (16 - n)
+    assert len(limbs_data) <= lpw
+    A = [list(a) for a in limbs_data]
+    ws, sb, pm, sg = windows(n, inv), stage_bits(n, inv), perms(n, inv), sigmas(n, inv)
+    assert sum(len(s) for s in sb) == n
+    stride = lds_slot_stride(n)
+    lds_words = lpw * stride
+    assert lds_words <= 33 * 1024 + 64
+    regs = [[None] * E for _ in range(THREADS)]
+    valid = [(t >> g) < len(A) for t in range(THREADS)]
+    for k, lo in enumerate(ws):
+        p = pm[k]
+        if k == 0:
+            for t in range(THREADS):
+                if not valid[t]:
+                    continue
+                slot, u = t >> g, t & ((1 << g) - 1)
+                for xw in range(E):
+                    i = vidx(lo, u, xw)
+                    assert 0 <= i < N
+                    v = A[slot][i]
+                    if not inv and scale is not None:
+                        v = v * scale[slot] % qs[slot]
+                    regs[t][reg_of(p, xw)] = v
+        # stages
+        for t in range(THREADS):
+            if not valid[t]:
+                continue
+            slot, u = t >> g, t & ((1 << g) - 1)
+            q = qs[slot]
+            tb = (1 << (n - E_BITS - lo)) + (u >> lo)
+            for j in sb[k]:
+                for xw in range(E):
+                    if (xw >> j) & 1:
+                        continue
+                    ti = (tb << (E_BITS - 1 - j)) + (xw >> (j + 1))
+                    assert 0 < ti < N
+                    w = tws[slot][ti]
+                    a, b = reg_of(p, xw), reg_of(p, xw | (1 << j))
+                    x, y = regs[t][a], regs[t][b]
+                    if not inv:
+                        y = y * w % q
+                        regs[t][a], regs[t][b] = (x + y) % q, (x - y) % q
+                    else:
+                        regs[t][a], regs[t][b] = (x + y) % q, (x - y) * w % q
+        if k == len(ws) - 1:
+            break
+        # exchange to round k+1 in two halves
+        s = sg[k]
+        lo_b, pb = ws[k + 1], pm[k + 1]
+        ja, jb = s - lo, s - lo_b
+        ra = p[ja]
+        assert pb[jb] == ra
+        for h in (0, 1):
+            lds = [None] * lds_words
+            written = [set() for _ in range(THREADS)]
+            for t in range(THREADS):
+                if not valid[t]:
+                    continue
+                slot, u = t >> g, t & ((1 << g) - 1)
+                for xw in range(E):
+                    if ((xw >> ja) & 1) != h:
+                        continue
+                    i = vidx(lo, u, xw)
+                    assert ((i >> s) & 1) == h
+                    c = compress(i, s)
+                    a = slot * stride + c + (c >> 5)
+                    assert lds[a] is None, "two writers of one LDS word"
+                    lds[a] = regs[t][reg_of(p, xw)]
+                    written[t].add(reg_of(p, xw))
+            for t in range(THREADS):
+                if not valid[t]:
+                    continue
+                slot, u = t >> g, t & ((1 << g) - 1)
+                for xw in range(E):
+                    if ((xw >> jb) & 1) != h:
+                        continue
+                    r = reg_of(pb, xw)
+                    assert r in written[t], "register read before its value was saved"
+                    i = vidx(lo_b, u, xw)
+                    c = compress(i, s)
+                    a = slot * stride + c + (c >> 5)
+                    assert lds[a] is not None, "read of an unwritten LDS word"
+                    regs[t][r] = lds[a]
+    lo = ws[-1]
+    p = pm[-1]
+    stored = [[0] * N for _ in A]
+    for t in range(THREADS):
+        if not valid[t]:
+            continue
+        slot, u = t >> g, t & ((1 << g) - 1)
+        for xw in range(E):
+            i = vidx(lo, u, xw)
+            v = regs[t][reg_of(p, xw)]
+            if inv and scale is not None:
+                v = v * scale[slot] % qs[slot]
+            A[slot][i] = v
+            stored[slot][i] += 1
+    assert all(c == 1 for s_ in stored for c in s_)
+    return A

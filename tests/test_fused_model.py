@@ -1,0 +1,37 @@
+"""CPU check of the fused limb kernel's schedule (tests/fused_model.py) for
+every supported size: bounds, exchange hazards, LDS ownership and the
+transform against the oracle."""
+import numpy as np
+import pytest
+
+import ntt_amd
+from tests import fused_model as fm
+
+
+@pytest.mark.parametrize("n", [10, 11, 12, 13, 14, 15, 16])
+def test_fused_schedule(oracle_mod, n):
+    lpw = 1 << (16 - n)
+    nl = min(lpw, 3)
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(n, nl)]
+    rng = np.random.default_rng(n)
+    xs, tws, itws, qs, dinv = [], [], [], [], []
+    for q, g in moduli:
+        rc, rt = oracle_mod.make_modulus_runtime(n, q, g)
+        tws.append([int(v) for v in oracle_mod.twiddle_table(n, q, rt["psi"])])
+        itws.append([int(v) for v in oracle_mod.twiddle_table(n, q, rt["psi_inv"])])
+        qs.append(q)
+        dinv.append(rt["degree_inv"])
+        xs.append([int(v) for v in rng.integers(0, q, 1 << n)])
+    fwd = fm.run(xs, tws, qs, n, inv=False)
+    ref = oracle_mod.forward(np.array(xs, np.uint32), n, moduli, 0)
+    assert np.array_equal(np.array(fwd, np.uint32), ref)
+    back = fm.run(fwd, itws, qs, n, inv=True, scale=dinv)
+    assert back == xs
+
+
+def test_fused_windows_cover_all_bits():
+    for n in range(10, 17):
+        for inv in (False, True):
+            bits = [b for k, r in enumerate(fm.stage_bits(n, inv)) for b in (x + fm.windows(n, inv)[k] for x in r)]
+            assert sorted(bits) == list(range(n))
+            assert bits == (sorted(bits) if inv else sorted(bits, reverse=True))

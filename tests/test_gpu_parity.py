@@ -216,3 +216,78 @@ def test_launch_errors_are_status_codes(env):
     with pytest.raises(ntt_amd.NTTError) as e:
         ctx.plan(17, [(974258177, 3)])        # q != 1 mod 2^18
     assert e.value.status == 2
+
+
+def test_autotune_result_is_valid_and_exact(env, tmp_path):
+    ntt_amd, gu, orc = env
+    logN, limbs, count = 12, 4, 8
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    ctx = ntt_amd.Context(0)
+    _print("case autotune")
+    r = ctx.autotune(logN, moduli, count=count, reps=2)
+    cfg = ntt_amd.TransformConfig.make(*r["cfg"][:2], *r["cfg"][2:])
+    assert ntt_amd.validate_config(logN, cfg) == 0
+    assert r["candidates"] > 10 and r["fwd_ms"] > 0 and r["inv_ms"] > 0
+    plan = ctx.plan(logN, moduli, 1)          # picks the tuned entry for count operands
+    xs = np.stack([gu.random_operand(logN, moduli, 40 + k) for k in range(count)])
+    A = gu.to_dev(xs)
+    plan.forward(A, count)
+    gu.sync()
+    got = gu.to_host(A).reshape(xs.shape)
+    for k in range(count):
+        assert np.array_equal(got[k], orc.forward(xs[k], logN, moduli, 1))
+    plan.inverse(A, count)
+    gu.sync()
+    assert np.array_equal(gu.to_host(A).reshape(xs.shape), xs)
+    path = str(tmp_path / "tune.txt")
+    ctx.save_tuning(path)
+    ctx2 = ntt_amd.Context(0)
+    ctx2.load_tuning(path)
+    ctx2.save_tuning(path + "2")
+    assert open(path).read() == open(path + "2").read()
+
+
+@pytest.mark.parametrize("logN", [10, 11, 12, 13, 14, 15, 16])
+@pytest.mark.parametrize("engine", [2, 1])
+def test_engines_all_sizes(env, logN, engine):
+    """Fused limb kernel and two-pass kernels, bit-exact; limb counts that
+    leave the last fused work item partly empty."""
+    ntt_amd, gu, orc = env
+    lpw = 1 << (16 - logN)
+    for limbs in sorted({1, 3, lpw + 1}):
+        moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+        for domain in (1, 0):
+            ctx = ntt_amd.Context(0)
+            plan = ctx.plan(logN, moduli, domain)
+            plan.set_engine(engine)
+            assert plan.engine(1) == engine
+            x = gu.random_operand(logN, moduli, 1000 + limbs)
+            A = gu.to_dev(x)
+            _print("case engine", engine, logN, limbs, domain)
+            plan.forward(A)
+            gu.sync()
+            assert np.array_equal(gu.to_host(A).reshape(x.shape), orc.forward(x, logN, moduli, domain))
+            plan.inverse(A)
+            gu.sync()
+            assert np.array_equal(gu.to_host(A).reshape(x.shape), x)
+
+
+def test_fused_many_operands_period(env):
+    """count operands as one job: limb l uses the table of limb l % limbs."""
+    ntt_amd, gu, orc = env
+    logN, limbs, count = 13, 5, 7
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    ctx = ntt_amd.Context(0)
+    plan = ctx.plan(logN, moduli, 1)
+    plan.set_engine(2)
+    xs = np.stack([gu.random_operand(logN, moduli, 70 + k) for k in range(count)])
+    A = gu.to_dev(xs)
+    _print("case fused-many")
+    plan.forward(A, count)
+    gu.sync()
+    got = gu.to_host(A).reshape(xs.shape)
+    for k in range(count):
+        assert np.array_equal(got[k], orc.forward(xs[k], logN, moduli, 1))
+    plan.inverse(A, count)
+    gu.sync()
+    assert np.array_equal(gu.to_host(A).reshape(xs.shape), xs)
