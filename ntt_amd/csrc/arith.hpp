@@ -68,4 +68,44 @@ __device__ __forceinline__ void gs_butterfly(uint32_t& x, uint32_t& y, uint32_t 
     y = mont_mul(sub_mod(u, v, q), w, q, q_inv);
 }
 
+// ---- lazy reduction, q < 2^30 (so 4q < 2^32) ------------------------------
+// Values are kept as representatives in [0, 4q) (forward) or [0, 2q)
+// (inverse) and made canonical once, at the end of the transform; the
+// arithmetic mod q, hence every canonical output, is unchanged.
+
+// Montgomery product without the final correction: for a < 2^32 and b < q,
+// t + m q < 2^64 and the result lies in [0, 2q).
+__device__ __forceinline__ uint32_t mont_mul_lazy(uint32_t a, uint32_t b, uint32_t q, uint32_t q_inv) {
+    uint64_t t = (uint64_t)a * b;
+    uint32_t m = (uint32_t)t * q_inv;
+    return (uint32_t)(((uint64_t)m * q + t) >> 32);
+}
+
+// [0, 4q) -> [0, q)
+__device__ __forceinline__ uint32_t canon_4q(uint32_t x, uint32_t q) {
+    x = min(x, x - 2 * q);
+    return min(x, x - q);
+}
+
+// Cooley-Tukey on x, y in [0, 4q): x is first brought into [0, 2q), then
+// (x + wy, x - wy + 2q), both in [0, 4q).
+__device__ __forceinline__ void ct_butterfly_lazy(uint32_t& x, uint32_t& y, uint32_t w, uint32_t q,
+                                                  uint32_t q_inv) {
+    const uint32_t q2 = 2 * q;
+    const uint32_t u = min(x, x - q2);
+    const uint32_t v = mont_mul_lazy(y, w, q, q_inv);
+    x = u + v;
+    y = u + q2 - v;
+}
+
+// Gentleman-Sande on x, y in [0, 2q): (x + y mod 2q, (x - y + 2q) w), both in [0, 2q).
+__device__ __forceinline__ void gs_butterfly_lazy(uint32_t& x, uint32_t& y, uint32_t w, uint32_t q,
+                                                  uint32_t q_inv) {
+    const uint32_t q2 = 2 * q;
+    const uint32_t s = x + y;
+    const uint32_t d = x + q2 - y;
+    x = min(s, s - q2);
+    y = mont_mul_lazy(d, w, q, q_inv);
+}
+
 }  // namespace ntt

@@ -157,6 +157,7 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
     size_t best_chunk = 0;
     double def_fwd = 0, def_inv = 0, fused_fwd = 0, fused_inv = 0;
     int engine = NTT_ENGINE_TWO_PASS;
+    bool fused_canon = false;
     if (rc == NTT_OK) {
         // chunking of multi-operand launches (whole transforms, both passes)
         double best_pair = 1e30;
@@ -180,15 +181,19 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
             }
         }
         // the fused limb kernel, where it applies
-        if (rc == NTT_OK && fused_supported(logN)) {
+        for (int lz = P.lazy ? 1 : 0; rc == NTT_OK && fused_supported(logN) && lz >= 0; --lz) {
             double f = 0, i = 0;
-            rc = time_median(s, reps, [&] { return plan_launch_fused(&P, buf, count, false, s); }, &f);
-            if (rc == NTT_OK) rc = time_median(s, reps, [&] { return plan_launch_fused(&P, buf, count, true, s); }, &i);
+            rc = time_median(s, reps, [&] { return plan_launch_fused(&P, buf, count, false, s, lz != 0); }, &f);
+            if (rc == NTT_OK)
+                rc = time_median(s, reps, [&] { return plan_launch_fused(&P, buf, count, true, s, lz != 0); }, &i);
             ++ncand;
-            fused_fwd = f;
-            fused_inv = i;
+            if (rc == NTT_OK && (fused_fwd == 0 || f + i < fused_fwd + fused_inv)) {
+                fused_fwd = f;
+                fused_inv = i;
+            }
             if (rc == NTT_OK && f + i < best_fwd + best_inv) {
                 engine = NTT_ENGINE_FUSED;
+                fused_canon = lz == 0;
                 best_fwd = f;
                 best_inv = i;
                 best_chunk = 0;
@@ -214,6 +219,7 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
     TunedEntry e;
     e.cfg = best;
     e.engine = engine;
+    e.fused_canon = fused_canon;
     e.chunk = best_chunk;
     e.fwd_ms = best_fwd;
     e.inv_ms = best_inv;

@@ -29,6 +29,7 @@ struct LimbConst;
 struct TunedEntry {
     ntt_transform_config cfg;
     int engine = NTT_ENGINE_TWO_PASS;   // NTT_ENGINE_TWO_PASS or NTT_ENGINE_FUSED
+    bool fused_canon = false;           // fused engine: canonical arithmetic measured faster
     size_t chunk = 0;          // operands per chunk, 0 = whole launch
     double fwd_ms = 0, inv_ms = 0;
 };
@@ -52,6 +53,7 @@ struct ntt_plan {
     bool cfg_fixed = false;                        // config given by the caller
     ntt_transform_config cfg{};
     int engine = NTT_ENGINE_AUTO;
+    bool lazy = false;                             // every q < 2^30: lazy-reduction kernels
     size_t chunk = 0;
     void* d_const = nullptr;                       // one allocation holding the arrays below
     const uint32_t* const* d_tw_fwd = nullptr;     // per-limb forward table pointers
@@ -74,7 +76,8 @@ TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool i
 int log2_bucket(size_t v);
 // Engine a launch of `count` operands runs on (resolves NTT_ENGINE_AUTO).
 int plan_engine(const ntt_plan* P, size_t count);
-int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream);
+int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
+                      bool allow_lazy = true);
 bool fused_supported(int logN);
 int device_cus();
 }  // namespace ntt

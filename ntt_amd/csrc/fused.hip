@@ -6,7 +6,7 @@
 
 #include "../../include/ntt_amd.h"
 #include "context.hpp"
-#include "fused_kernel.hpp"
+#include "fused_types.hpp"
 #include "host_math.hpp"
 #include "launch.hpp"
 
@@ -32,41 +32,25 @@ uint32_t fused_items(int logN, size_t limbs) {
     return (uint32_t)((limbs + lpw - 1) / lpw);
 }
 
-using FusedFn = void (*)(FusedParams);
 
-static FusedFn fused_fn(int n, bool inv) {
-    switch (n * 2 + (inv ? 1 : 0)) {
-        case 20: return fused_kernel<10, false>;
-        case 21: return fused_kernel<10, true>;
-        case 22: return fused_kernel<11, false>;
-        case 23: return fused_kernel<11, true>;
-        case 24: return fused_kernel<12, false>;
-        case 25: return fused_kernel<12, true>;
-        case 26: return fused_kernel<13, false>;
-        case 27: return fused_kernel<13, true>;
-        case 28: return fused_kernel<14, false>;
-        case 29: return fused_kernel<14, true>;
-        case 30: return fused_kernel<15, false>;
-        case 31: return fused_kernel<15, true>;
-        case 32: return fused_kernel<16, false>;
-        case 33: return fused_kernel<16, true>;
-        default: return nullptr;
-    }
-}
+FusedFn fused_fn_lazy(int n, bool inv);    // fused_lazy.hip
+FusedFn fused_fn_canon(int n, bool inv);   // fused_canon.hip
+
+static FusedFn fused_fn(int n, bool inv, bool lazy) { return lazy ? fused_fn_lazy(n, inv) : fused_fn_canon(n, inv); }
 
 // Launches the persistent kernel of size 2^n / direction `inv`; every job in
 // p has that size and direction.
-int launch_fused(int n, bool inv, const FusedParams& p, hipStream_t stream, unsigned grid_limit) {
+int launch_fused(int n, bool inv, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit) {
     if (p.total_items == 0) return NTT_OK;
-    FusedFn fn = fused_fn(n, inv);
+    FusedFn fn = fused_fn(n, inv, lazy);
     if (!fn) return NTT_ERR_CONFIG;
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
-    const size_t lds = fused::LDS_WORDS * sizeof(uint32_t);
+    const size_t lds = FUSED_LDS_WORDS * sizeof(uint32_t);
     std::call_once(attr_once, [&] {
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
-            for (int d = 0; d < 2 && attr_err == hipSuccess; ++d)
-                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn(m, d != 0)),
+            for (int d = 0; d < 4 && attr_err == hipSuccess; ++d)
+                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn(m, d & 1, d >> 1)),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     });
     if (attr_err != hipSuccess) return record_hip_error(attr_err, "hipFuncSetAttribute(fused)");
@@ -75,13 +59,14 @@ int launch_fused(int n, bool inv, const FusedParams& p, hipStream_t stream, unsi
     if (p.total_items < grid) grid = p.total_items;
     hipError_t stale = hipGetLastError();
     if (stale != hipSuccess) record_hip_error(stale, "error pending before launch (cleared)");
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(fused::THREADS), lds, stream, p);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(FUSED_THREADS), lds, stream, p);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? NTT_OK : record_hip_error(e, "fused kernel launch");
 }
 
 // `count` operands of a plan as one job (limb l uses table l % limbs).
-int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream) {
+int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
+                      bool allow_lazy) {
     if (!fused_supported(P->logN)) return NTT_ERR_CONFIG;
     if (reinterpret_cast<uintptr_t>(A) & 15) return NTT_ERR_INVALID_ARGUMENT;
     const size_t limbs = P->limbs * count;
@@ -99,7 +84,7 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.njobs = 0;
     p.total_items = fused_items(P->logN, limbs);
     p.counter = nullptr;
-    return launch_fused(P->logN, inverse, p, stream, 0);
+    return launch_fused(P->logN, inverse, P->lazy && allow_lazy, p, stream, 0);
 }
 
 }  // namespace ntt

@@ -44,6 +44,14 @@ constexpr int E = 64;
 constexpr int THREADS = 1024;
 constexpr uint32_t LDS_WORDS = 32768 + 1024;
 
+// Global-memory pointers: loads through them use an SGPR base plus a 32-bit
+// VGPR offset instead of a 64-bit flat address per lane.
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4 gu4;
+typedef __attribute__((address_space(1))) const u32x4 gcu4;
+
 __host__ __device__ constexpr int nrounds(int n) { return n >= 12 ? 3 : 2; }
 
 __host__ __device__ constexpr int win_lo(int n, bool inv, int k) {
@@ -113,8 +121,8 @@ __device__ __forceinline__ uint32_t lds_index(uint32_t i) {
     return c + (c >> 5);
 }
 
-template <int n, bool INV, int K>
-__device__ __forceinline__ void stages(uint32_t (&v)[E], const uint32_t* __restrict__ tw, uint32_t u, uint32_t q,
+template <int n, bool INV, bool LAZY, int K>
+__device__ __forceinline__ void stages(uint32_t (&v)[E], gcu32* __restrict__ tw, uint32_t u, uint32_t q,
                                        uint32_t q_inv) {
     constexpr int LO = win_lo(n, INV, K);
     constexpr int JLO = stage_jlo(n, INV, K), JHI = stage_jhi(n, INV, K);
@@ -128,10 +136,16 @@ __device__ __forceinline__ void stages(uint32_t (&v)[E], const uint32_t* __restr
                 constexpr int x = ((p >> j) << (j + 1)) | (p & ((1 << j) - 1));
                 constexpr int a = REG<n, INV, K, x>, b = REG<n, INV, K, x | (1 << j)>;
                 const uint32_t w = tw[(TB << (EB - 1 - j)) + (x >> (j + 1))];
-                if (INV)
+                if (LAZY) {
+                    if (INV)
+                        gs_butterfly_lazy(v[a], v[b], w, q, q_inv);
+                    else
+                        ct_butterfly_lazy(v[a], v[b], w, q, q_inv);
+                } else if (INV) {
                     gs_butterfly(v[a], v[b], w, q, q_inv);
-                else
+                } else {
                     ct_butterfly(v[a], v[b], w, q, q_inv);
+                }
             });
         }
     });
@@ -161,7 +175,7 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[E], uint32_t* L, uint32_t
     });
 }
 
-template <int n, bool INV>
+template <int n, bool INV, bool LAZY>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds) {
     constexpr int G_BITS = n - EB;
     constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 6));
@@ -178,8 +192,8 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     const bool valid = limb < job.limbs;
     const uint32_t lsafe = valid ? limb : first_limb;
     const uint32_t per = lsafe % job.period;
-    uint32_t* A = job.data + ((size_t)lsafe << n);
-    const uint32_t* tw = job.tw[per];
+    gu32* A = (gu32*)(job.data + ((size_t)lsafe << n));
+    gcu32* tw = (gcu32*)job.tw[per];
     const LimbConst c = job.lc[per];
     uint32_t* L = lds + slot * STRIDE;
 
@@ -187,10 +201,10 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     // ---- load (round 0 layout) ----
     if (valid) {
         if (LO0 == 0) {
-            const uint4* src = reinterpret_cast<const uint4*>(A + vidx<LO0>(u, 0));
+            gcu4* src = (gcu4*)(A + vidx<LO0>(u, 0));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
                 constexpr int x4 = decltype(X4)::value;
-                const uint4 w = src[x4];
+                const u32x4 w = src[x4];
                 v[REG<n, INV, 0, 4 * x4>] = w.x;
                 v[REG<n, INV, 0, 4 * x4 + 1>] = w.y;
                 v[REG<n, INV, 0, 4 * x4 + 2>] = w.z;
@@ -204,19 +218,19 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         }
         if (!INV && (c.flags & 1)) {
 #pragma unroll
-            for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], c.scale, c.q, c.q_inv);
+            for (int x = 0; x < E; ++x) v[x] = LAZY ? mont_mul_lazy(v[x], c.scale, c.q, c.q_inv) : mont_mul(v[x], c.scale, c.q, c.q_inv);
         }
     } else {
 #pragma unroll
         for (int x = 0; x < E; ++x) v[x] = 0;
     }
 
-    stages<n, INV, 0>(v, tw, u, c.q, c.q_inv);
+    stages<n, INV, LAZY, 0>(v, tw, u, c.q, c.q_inv);
     exchange<n, INV, 0>(v, L, u, valid);
-    stages<n, INV, 1>(v, tw, u, c.q, c.q_inv);
+    stages<n, INV, LAZY, 1>(v, tw, u, c.q, c.q_inv);
     if constexpr (NR == 3) {
         exchange<n, INV, 1>(v, L, u, valid);
-        stages<n, INV, 2>(v, tw, u, c.q, c.q_inv);
+        stages<n, INV, LAZY, 2>(v, tw, u, c.q, c.q_inv);
     }
 
     // ---- store (last round layout) ----
@@ -224,13 +238,17 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         if (INV && (c.flags & 1)) {
 #pragma unroll
             for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], c.scale, c.q, c.q_inv);
+        } else if (LAZY) {
+            // forward values lie in [0, 4q), unscaled inverse ones in [0, 2q)
+#pragma unroll
+            for (int x = 0; x < E; ++x) v[x] = canon_4q(v[x], c.q);
         }
         if (LOL == 0) {
-            uint4* dst = reinterpret_cast<uint4*>(A + vidx<LOL>(u, 0));
+            gu4* dst = (gu4*)(A + vidx<LOL>(u, 0));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
                 constexpr int x4 = decltype(X4)::value;
-                dst[x4] = make_uint4(v[REG<n, INV, NR - 1, 4 * x4>], v[REG<n, INV, NR - 1, 4 * x4 + 1>],
-                                     v[REG<n, INV, NR - 1, 4 * x4 + 2>], v[REG<n, INV, NR - 1, 4 * x4 + 3>]);
+                dst[x4] = u32x4{v[REG<n, INV, NR - 1, 4 * x4>], v[REG<n, INV, NR - 1, 4 * x4 + 1>],
+                                v[REG<n, INV, NR - 1, 4 * x4 + 2>], v[REG<n, INV, NR - 1, 4 * x4 + 3>]};
             });
         } else {
             static_for<E>([&](auto X) __attribute__((always_inline)) {
@@ -246,7 +264,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 // Persistent loop over work items of jobs of one size 2^n and one direction;
 // an item is 2^(16-n) consecutive limbs of one job.  Every thread of the
 // workgroup runs every barrier of every item.
-template <int n, bool INV>
+template <int n, bool INV, bool LAZY>
 __global__ __launch_bounds__(1024, 1) void fused_kernel(FusedParams p) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_next;
@@ -274,7 +292,7 @@ __global__ __launch_bounds__(1024, 1) void fused_kernel(FusedParams p) {
             }
             job = p.jobs[lo];
         }
-        fused::run_item<n, INV>(job, (item - job.item_begin) << (16 - n), lds);
+        fused::run_item<n, INV, LAZY>(job, (item - job.item_begin) << (16 - n), lds);
     }
 }
 

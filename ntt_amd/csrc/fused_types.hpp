@@ -6,6 +6,13 @@
 
 namespace ntt {
 
+// Launch shape of the fused kernel: 1024 threads, 32K + 1K words of LDS.
+constexpr unsigned FUSED_THREADS = 1024;
+constexpr uint32_t FUSED_LDS_WORDS = 32768 + 1024;
+
+struct FusedParams;
+using FusedFn = void (*)(FusedParams);
+
 struct LimbConst {
     uint32_t q;
     uint32_t q_inv;

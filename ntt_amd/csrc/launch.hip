@@ -19,25 +19,30 @@ namespace ntt {
 
 using KernelFn = void (*)(PhaseParams);
 
-template <int S>
-static KernelFn pick_s(bool inv, bool col, bool twconv) {
-    if (!inv && col && !twconv) return phase_kernel<S, false, true, false>;
-    if (!inv && col && twconv) return phase_kernel<S, false, true, true>;
-    if (!inv && !col && !twconv) return phase_kernel<S, false, false, false>;
-    if (!inv && !col && twconv) return phase_kernel<S, false, false, true>;
-    if (inv && col && !twconv) return phase_kernel<S, true, true, false>;
-    if (inv && col && twconv) return phase_kernel<S, true, true, true>;
-    if (inv && !col && !twconv) return phase_kernel<S, true, false, false>;
-    return phase_kernel<S, true, false, true>;
+template <int S, int MODE>
+static KernelFn pick_m(bool inv, bool col) {
+    if (!inv && col) return phase_kernel<S, false, true, MODE>;
+    if (!inv && !col) return phase_kernel<S, false, false, MODE>;
+    if (inv && col) return phase_kernel<S, true, true, MODE>;
+    return phase_kernel<S, true, false, MODE>;
 }
 
-static KernelFn pick(int S, bool inv, bool col, bool twconv) {
+template <int S>
+static KernelFn pick_s(bool inv, bool col, int mode) {
+    switch (mode) {
+        case MODE_CANON: return pick_m<S, MODE_CANON>(inv, col);
+        case MODE_TWCONV: return pick_m<S, MODE_TWCONV>(inv, col);
+        default: return pick_m<S, MODE_LAZY>(inv, col);
+    }
+}
+
+static KernelFn pick(int S, bool inv, bool col, int mode) {
     switch (S) {
-        case 1: return pick_s<1>(inv, col, twconv);
-        case 2: return pick_s<2>(inv, col, twconv);
-        case 3: return pick_s<3>(inv, col, twconv);
-        case 4: return pick_s<4>(inv, col, twconv);
-        case 5: return pick_s<5>(inv, col, twconv);
+        case 1: return pick_s<1>(inv, col, mode);
+        case 2: return pick_s<2>(inv, col, mode);
+        case 3: return pick_s<3>(inv, col, mode);
+        case 4: return pick_s<4>(inv, col, mode);
+        case 5: return pick_s<5>(inv, col, mode);
         default: return nullptr;
     }
 }
@@ -63,7 +68,7 @@ size_t device_max_lds() {
     return g_max_lds;
 }
 
-static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, bool twconv, const PhaseParams& base,
+static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, int mode, const PhaseParams& base,
                        hipStream_t stream) {
     if (pc.radix_stages == 0) return NTT_OK;
     const int R = pc.radix_stages, S = pc.stage_merging, W = pc.warp_batching;
@@ -76,7 +81,7 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, bool twco
     p.xcd_remap = (base.xcd_remap && blocks % 8 == 0) ? 1 : 0;
     const unsigned threads = 1u << (R - S + W);
     const size_t lds = lds_bytes_for((size_t)1 << (R + W));
-    KernelFn fn = pick(S, inv, col, twconv);
+    KernelFn fn = pick(S, inv, col, mode);
     if (!fn) return NTT_ERR_CONFIG;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),
@@ -131,20 +136,23 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
     scaled.scale = L.scale;
     scaled.scale_mode = L.scale_mode;
 
+    const int mode = L.twconv ? MODE_TWCONV : (L.lazy ? MODE_LAZY : MODE_CANON);
     if (!L.inverse) {
         const ntt_phase_config &c1 = L.cfg.ntt_phase1, &c2 = L.cfg.ntt_phase2;
         const bool first_is_col = c1.radix_stages > 0;
-        if ((pass_mask & 1) && (rc = launch_pass(c1, false, true, L.twconv, first_is_col ? scaled : none, stream)))
-            return rc;
-        if ((pass_mask & 2) && (rc = launch_pass(c2, false, false, L.twconv, first_is_col ? none : scaled, stream)))
-            return rc;
+        PhaseParams a = first_is_col ? scaled : none, b = first_is_col ? none : scaled;
+        a.final_pass = c2.radix_stages == 0;
+        b.final_pass = 1;
+        if ((pass_mask & 1) && (rc = launch_pass(c1, false, true, mode, a, stream))) return rc;
+        if ((pass_mask & 2) && (rc = launch_pass(c2, false, false, mode, b, stream))) return rc;
     } else {
         const ntt_phase_config &c1 = L.cfg.intt_phase1, &c2 = L.cfg.intt_phase2;
         const bool last_is_col = c2.radix_stages > 0;
-        if ((pass_mask & 1) && (rc = launch_pass(c1, true, false, L.twconv, last_is_col ? none : scaled, stream)))
-            return rc;
-        if ((pass_mask & 2) && (rc = launch_pass(c2, true, true, L.twconv, last_is_col ? scaled : none, stream)))
-            return rc;
+        PhaseParams a = last_is_col ? none : scaled, b = last_is_col ? scaled : none;
+        a.final_pass = c2.radix_stages == 0;
+        b.final_pass = 1;
+        if ((pass_mask & 1) && (rc = launch_pass(c1, true, false, mode, a, stream))) return rc;
+        if ((pass_mask & 2) && (rc = launch_pass(c2, true, true, mode, b, stream))) return rc;
     }
     return NTT_OK;
 }

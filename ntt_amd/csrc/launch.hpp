@@ -23,6 +23,7 @@ struct TransformLaunch {
     size_t count;              // operands laid out back to back
     bool inverse;
     bool twconv;               // table is normal domain: convert each twiddle on load
+    bool lazy;                 // every q < 2^30: lazy-reduction arithmetic
     int xcd_remap;
     // Operands per chunk (0 = all): both passes run chunk by chunk, so the
     // intermediate between the passes of one chunk can stay in the caches.

@@ -57,6 +57,14 @@ struct PhaseParams {
     uint32_t limbs;           // limbs per operand
     uint32_t count;           // operands
     int xcd_remap;            // 1: group workgroups of the same limb index on one XCD
+    int final_pass;           // 1: this pass ends the transform (outputs made canonical)
+};
+
+// Arithmetic variants of the pass kernels.
+enum PassMode : int {
+    MODE_CANON = 0,    // canonical residues, q < 2^31, Montgomery-form table
+    MODE_TWCONV = 1,   // canonical, normal-domain table converted on load (v03 drop-in)
+    MODE_LAZY = 2,     // lazy residues, q < 2^30, Montgomery-form table
 };
 
 __device__ __forceinline__ uint32_t lds_pad(uint32_t i) { return i + (i >> 5); }
@@ -70,7 +78,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Stages on window bits [jlo, jhi] (forward: from jhi down; inverse: from jlo up).
-template <int S, bool INV, bool TWCONV>
+template <int S, bool INV, int MODE>
 __device__ __forceinline__ void round_stages(uint32_t (&v)[1 << S], const uint32_t* __restrict__ tw,
                                              uint32_t TB, int jlo, int jhi, uint32_t q,
                                              uint32_t q_inv, uint32_t r2) {
@@ -84,18 +92,24 @@ __device__ __forceinline__ void round_stages(uint32_t (&v)[1 << S], const uint32
         for (int x = 0; x < E; ++x) {
             if (x & (1 << j)) continue;
             uint32_t w = tw[base + (x >> (j + 1))];
-            if (TWCONV) w = mont_mul(w, r2, q, q_inv);
-            if (INV)
+            if (MODE == MODE_TWCONV) w = mont_mul(w, r2, q, q_inv);
+            if (MODE == MODE_LAZY) {
+                if (INV)
+                    gs_butterfly_lazy(v[x], v[x + (1 << j)], w, q, q_inv);
+                else
+                    ct_butterfly_lazy(v[x], v[x + (1 << j)], w, q, q_inv);
+            } else if (INV) {
                 gs_butterfly(v[x], v[x + (1 << j)], w, q, q_inv);
-            else
+            } else {
                 ct_butterfly(v[x], v[x + (1 << j)], w, q, q_inv);
+            }
         }
     }
 }
 
 // One pass.  COL: column pass (outer stages; forward phase 1 / inverse phase 2).
 // !COL: row pass (inner stages; forward phase 2 / inverse phase 1).
-template <int S, bool INV, bool COL, bool TWCONV>
+template <int S, bool INV, bool COL, int MODE>
 __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     constexpr int E = 1 << S;
     extern __shared__ uint32_t lds[];
@@ -120,7 +134,7 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     const uint32_t q = p.moduli[li];
     const uint32_t q_inv = p.q_inv ? p.q_inv[li] : neg_inv_mod_2_32(q);
     uint32_t r2 = 0;
-    if (TWCONV || p.scale_mode == SCALE_TO_MONT) r2 = r2_mod(q);
+    if (MODE == MODE_TWCONV || p.scale_mode == SCALE_TO_MONT) r2 = r2_mod(q);
     uint32_t mult = 0;
     if (p.scale_mode != SCALE_NONE) {
         const uint32_t s = p.scale[li];
@@ -209,16 +223,17 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
             gload(lo, v);
             if (!INV && scaled) {
 #pragma unroll
-                for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], mult, q, q_inv);
+                for (int x = 0; x < E; ++x)
+                    v[x] = MODE == MODE_LAZY ? mont_mul_lazy(v[x], mult, q, q_inv) : mont_mul(v[x], mult, q, q_inv);
             }
         } else {
 #pragma unroll
             for (int x = 0; x < E; ++x) v[x] = lds[laddr(vidx(lo, x))];
         }
         if (INV)
-            round_stages<S, true, TWCONV>(v, tw, twbase(lo), last ? S - rem : 0, S - 1, q, q_inv, r2);
+            round_stages<S, true, MODE>(v, tw, twbase(lo), last ? S - rem : 0, S - 1, q, q_inv, r2);
         else
-            round_stages<S, false, TWCONV>(v, tw, twbase(lo), 0, last ? rem - 1 : S - 1, q, q_inv, r2);
+            round_stages<S, false, MODE>(v, tw, twbase(lo), 0, last ? rem - 1 : S - 1, q, q_inv, r2);
         if (!last) {
 #pragma unroll
             for (int x = 0; x < E; ++x) lds[laddr(vidx(lo, x))] = v[x];
@@ -227,6 +242,10 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
             if (INV && scaled) {
 #pragma unroll
                 for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], mult, q, q_inv);
+            } else if (MODE == MODE_LAZY && p.final_pass) {
+                // forward values lie in [0, 4q), unscaled inverse ones in [0, 2q)
+#pragma unroll
+                for (int x = 0; x < E; ++x) v[x] = canon_4q(v[x], q);
             }
             gstore(lo, v);
         }

@@ -78,6 +78,8 @@ int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, siz
         if ((rc = ctx->twiddles.get(rt.qi, rt.psi, logN, false, &twf[l]))) return rc;
         if ((rc = ctx->twiddles.get(rt.qi, rt.psi_inv, logN, true, &twi[l]))) return rc;
     }
+    P->lazy = true;
+    for (size_t l = 0; l < limbs; ++l) P->lazy = P->lazy && q[l] < (1u << 30);
     const size_t u32b = limbs * sizeof(uint32_t), ptrb = limbs * sizeof(void*);
     const size_t lcb = limbs * sizeof(LimbConst);
     if (hipMalloc(&P->d_const, 2 * lcb + 4 * u32b + 2 * ptrb) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
@@ -123,6 +125,7 @@ TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool i
     L.count = count;
     L.inverse = inverse;
     L.twconv = false;
+    L.lazy = P->lazy;
     L.xcd_remap = 1;
     L.cfg = P->cfg;
     L.chunk = P->chunk;
@@ -221,8 +224,12 @@ int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out) {
 
 static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse, void* stream) {
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
-    if (plan_engine(plan, count) == NTT_ENGINE_FUSED)
-        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream);
+    if (plan_engine(plan, count) == NTT_ENGINE_FUSED) {
+        TunedEntry t;
+        const bool canon = !plan->cfg_fixed && plan->ctx->tuned_lookup(plan->logN, plan->limbs * count, &t) &&
+                           t.fused_canon;
+        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, !canon);
+    }
     return launch_transform(plan_launch(plan, A_dev, count, inverse), (hipStream_t)stream);
 }
 

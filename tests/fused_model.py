@@ -93,8 +93,14 @@ def lds_slot_stride(n: int) -> int:
     return half + (half >> 5)
 
 
-def run(limbs_data, tws, qs, n: int, inv: bool, scale=None):
-    """limbs_data: list of 2^(16-n) limbs (lists), modified copies returned."""
+def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False):
+    """limbs_data: list of 2^(16-n) limbs (lists), modified copies returned.
+
+    lazy=False: canonical arithmetic on normal-domain values (scale is a plain
+    multiplier).  lazy=True: the kernels' 32-bit lazy arithmetic
+    (tests/lazy_arith.py) with Montgomery-form twiddles; scale is the
+    Montgomery multiplier applied on load (forward) or store (inverse)."""
+    from tests import lazy_arith as la
     N = 1 << n
     g = n - E_BITS
     lpw = 1 << (16 - n)
@@ -119,7 +125,8 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None):
                     assert 0 <= i < N
                     v = A[slot][i]
                     if not inv and scale is not None:
-                        v = v * scale[slot] % qs[slot]
+                        v = la.mont_lazy(v, scale[slot], qs[slot], la.qinv_of(qs[slot])) if lazy \
+                            else v * scale[slot] % qs[slot]
                     regs[t][reg_of(p, xw)] = v
         # stages
         for t in range(THREADS):
@@ -127,6 +134,7 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None):
                 continue
             slot, u = t >> g, t & ((1 << g) - 1)
             q = qs[slot]
+            qinv = la.qinv_of(q)
             tb = (1 << (n - E_BITS - lo)) + (u >> lo)
             for j in sb[k]:
                 for xw in range(E):
@@ -137,7 +145,10 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None):
                     w = tws[slot][ti]
                     a, b = reg_of(p, xw), reg_of(p, xw | (1 << j))
                     x, y = regs[t][a], regs[t][b]
-                    if not inv:
+                    if lazy:
+                        wm = la.to_mont(w, q)
+                        regs[t][a], regs[t][b] = (la.gs_lazy if inv else la.ct_lazy)(x, y, wm, q, qinv)
+                    elif not inv:
                         y = y * w % q
                         regs[t][a], regs[t][b] = (x + y) % q, (x - y) % q
                     else:
@@ -191,7 +202,9 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None):
         for xw in range(E):
             i = vidx(lo, u, xw)
             v = regs[t][reg_of(p, xw)]
-            if inv and scale is not None:
+            if lazy:
+                v = la.mont(v, scale[slot], qs[slot], la.qinv_of(qs[slot])) if inv else la.canon_4q(v, qs[slot])
+            elif inv and scale is not None:
                 v = v * scale[slot] % qs[slot]
             A[slot][i] = v
             stored[slot][i] += 1

@@ -22,8 +22,14 @@ def lds_words(R: int, W: int) -> int:
     return w + (w >> 5) + 1
 
 
-def run_pass(A, tw, q, n, R, S, W, inv, col, mult=None):
-    """One pass over limb A (list of N ints, modified in place)."""
+def run_pass(A, tw, q, n, R, S, W, inv, col, mult=None, lazy=False, final_pass=False):
+    """One pass over limb A (list of N ints, modified in place).
+
+    lazy: the kernels' 32-bit lazy arithmetic (tests/lazy_arith.py) with
+    Montgomery-form twiddles; mult is then the Montgomery multiplier, and
+    `final_pass` marks the pass that must leave canonical values."""
+    from tests import lazy_arith as lz
+    qinv = lz.qinv_of(q)
     N = 1 << n
     E = 1 << S
     lane_bits = R - S
@@ -85,7 +91,7 @@ def run_pass(A, tw, q, n, R, S, W, inv, col, mult=None):
                         loaded[a] += 1
                         v[x] = A[a]
                         if not inv and mult is not None:
-                            v[x] = v[x] * mult % q
+                            v[x] = lz.mont_lazy(v[x], mult, q, qinv) if lazy else v[x] * mult % q
                     else:
                         la = laddr(c, rib, vi)
                         assert lds[la] is not None, "read of an LDS word nobody wrote"
@@ -105,7 +111,9 @@ def run_pass(A, tw, q, n, R, S, W, inv, col, mult=None):
                         assert 0 < ti < N
                         w = tw[ti]
                         y = x + (1 << j)
-                        if inv:
+                        if lazy:
+                            v[x], v[y] = (lz.gs_lazy if inv else lz.ct_lazy)(v[x], v[y], lz.to_mont(w, q), q, qinv)
+                        elif inv:
                             u, vv = v[x], v[y]
                             v[x], v[y] = (u + vv) % q, (u - vv) * w % q
                         else:
@@ -120,7 +128,12 @@ def run_pass(A, tw, q, n, R, S, W, inv, col, mult=None):
                         lds[la] = v[x]
                 else:
                     for x in range(E):
-                        if inv and mult is not None:
+                        if lazy:
+                            if inv and mult is not None:
+                                v[x] = lz.mont(v[x], mult, q, qinv)
+                            elif final_pass and not inv:
+                                v[x] = lz.canon_4q(v[x], q)
+                        elif inv and mult is not None:
                             v[x] = v[x] * mult % q
                         a = gaddr(lane, c, g, vidx(lane, lo, x))
                         stored[a] += 1
@@ -128,22 +141,25 @@ def run_pass(A, tw, q, n, R, S, W, inv, col, mult=None):
     assert all(c == 1 for c in loaded) and all(c == 1 for c in stored)
 
 
-def forward(a, q, psi_tw, n, cfg):
+def forward(a, q, psi_tw, n, cfg, lazy=False, mult=None):
     """cfg = ((R1,S1,W1),(R2,S2,W2),...) as TransformConfig.astuple()."""
     A = list(a)
     (r1, s1, w1), (r2, s2, w2) = cfg[0], cfg[1]
     if r1:
-        run_pass(A, psi_tw, q, n, r1, s1, w1, inv=False, col=True)
+        run_pass(A, psi_tw, q, n, r1, s1, w1, inv=False, col=True, mult=mult, lazy=lazy, final_pass=not r2)
     if r2:
-        run_pass(A, psi_tw, q, n, r2, s2, w2, inv=False, col=False)
+        run_pass(A, psi_tw, q, n, r2, s2, w2, inv=False, col=False, mult=None if r1 else mult, lazy=lazy,
+                 final_pass=True)
     return A
 
 
-def inverse(a, q, ipsi_tw, n, cfg, ninv):
+def inverse(a, q, ipsi_tw, n, cfg, ninv, lazy=False):
+    """ninv: N^-1 (plain multiplier), or with lazy the Montgomery multiplier."""
     A = list(a)
     (r1, s1, w1), (r2, s2, w2) = cfg[2], cfg[3]
     if r1:
-        run_pass(A, ipsi_tw, q, n, r1, s1, w1, inv=True, col=False, mult=None if r2 else ninv)
+        run_pass(A, ipsi_tw, q, n, r1, s1, w1, inv=True, col=False, mult=None if r2 else ninv, lazy=lazy,
+                 final_pass=not r2)
     if r2:
-        run_pass(A, ipsi_tw, q, n, r2, s2, w2, inv=True, col=True, mult=ninv)
+        run_pass(A, ipsi_tw, q, n, r2, s2, w2, inv=True, col=True, mult=ninv, lazy=lazy, final_pass=True)
     return A

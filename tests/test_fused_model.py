@@ -27,6 +27,16 @@ def test_fused_schedule(oracle_mod, n):
     assert np.array_equal(np.array(fwd, np.uint32), ref)
     back = fm.run(fwd, itws, qs, n, inv=True, scale=dinv)
     assert back == xs
+    # the lazy 32-bit arithmetic: plain-domain forward, inverse with the
+    # Montgomery multiplier to_mont(N^-1)
+    lfwd = fm.run(xs, tws, qs, n, inv=False, lazy=True)
+    assert np.array_equal(np.array(lfwd, np.uint32), ref)
+    mdinv = [(d << 32) % q for d, q in zip(dinv, qs)]
+    assert fm.run(lfwd, itws, qs, n, inv=True, scale=mdinv, lazy=True) == xs
+    # Montgomery-domain forward: x R^2 on load gives to_mont(NTT(x))
+    r2 = [pow(2, 64, q) for q in qs]
+    mfwd = fm.run(xs, tws, qs, n, inv=False, scale=r2, lazy=True)
+    assert mfwd == [[(v << 32) % q for v in row] for row, q in zip(ref.tolist(), qs)]
 
 
 def test_fused_windows_cover_all_bits():

@@ -291,3 +291,25 @@ def test_fused_many_operands_period(env):
     plan.inverse(A, count)
     gu.sync()
     assert np.array_equal(gu.to_host(A).reshape(xs.shape), xs)
+
+
+@pytest.mark.parametrize("engine", [2, 1])
+def test_engines_canonical_arithmetic(env, engine):
+    """default_moduli() has q >= 2^30, so plans use canonical (not lazy)
+    arithmetic; both engines stay bit-exact."""
+    ntt_amd, gu, orc = env
+    logN = 16
+    moduli = [tuple(m) for m in ntt_amd.default_moduli()]
+    for domain in (1, 0):
+        ctx = ntt_amd.Context(0)
+        plan = ctx.plan(logN, moduli, domain)
+        plan.set_engine(engine)
+        x = orc.fill_random_limbs(logN, moduli, 1234)
+        A = gu.to_dev(x)
+        _print("case canonical engine", engine, domain)
+        plan.forward(A)
+        gu.sync()
+        assert np.array_equal(gu.to_host(A).reshape(x.shape), orc.forward(x, logN, moduli, domain))
+        plan.inverse(A)
+        gu.sync()
+        assert np.array_equal(gu.to_host(A).reshape(x.shape), x)

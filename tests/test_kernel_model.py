@@ -34,10 +34,14 @@ def test_schedule_all_configs(oracle_mod, logN):
     ref = oracle_mod.forward(np.array([x], np.uint32), logN, [(q, g)], 0)[0].tolist()
     cfgs = valid_configs(logN)
     assert cfgs
+    mdinv = (rt["degree_inv"] << 32) % q
     for cfg in cfgs:
         fwd = kernel_model.forward(x, q, tw, logN, cfg)
         assert fwd == ref, cfg
         assert kernel_model.inverse(fwd, q, itw, logN, cfg, rt["degree_inv"]) == x, cfg
+        lf = kernel_model.forward(x, q, tw, logN, cfg, lazy=True)
+        assert lf == ref, cfg
+        assert kernel_model.inverse(lf, q, itw, logN, cfg, mdinv, lazy=True) == x, cfg
 
 
 @pytest.mark.parametrize("logN,cfg", [
