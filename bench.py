@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the per-transform and single-transform legs")
     ap.add_argument("--no-tune", action="store_true", help="use the default phase config instead of the autotuner's")
+    ap.add_argument("--engine", choices=["auto", "fused", "two_pass"], default="auto",
+                    help="force a kernel family for the batched leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one device-to-device copy of the batch (known bytes, for counter calibration)")
     return ap.parse_args()
 
 
@@ -122,6 +126,9 @@ def main():
         tuning["seconds"] = round(time.perf_counter() - t0, 2)
         log(f"autotune: {json.dumps(tuning)}")
     plan = ctx.plan(logN, moduli, ntt_amd.MONTGOMERY)
+    if args.engine != "auto":
+        plan.set_engine({"fused": ntt_amd.ENGINE_FUSED, "two_pass": ntt_amd.ENGINE_TWO_PASS}[args.engine])
+    engine_used = {1: "two_pass", 2: "fused"}.get(plan.engine(B), "?")
     cfg = ntt_amd.TransformConfig.make(*tuning["batch"]["cfg"]) if tuning else plan.config
     words_per_op = limbs * N
     pool = dev.DeviceBuffer(4 * words_per_op * B)
@@ -131,6 +138,13 @@ def main():
     dev.synchronize()
     probe = [0, B - 1]
     before = {k: pool.numpy(np.uint32, offset_bytes=4 * words_per_op * k, nbytes=4 * words_per_op) for k in probe}
+
+    if args.copy_calibration:
+        scratch = dev.DeviceBuffer(4 * words_per_op * B)
+        scratch.copy_from(pool, stream=stream)
+        stream.synchronize()
+        log(f"copy calibration: {4 * words_per_op * B} bytes read and written")
+        del scratch
 
     bfly_per_transform = ntt_amd.butterflies(logN, limbs)
     bytes_per_transform = ntt_amd.algorithmic_bytes(logN, limbs)
@@ -260,6 +274,7 @@ def main():
                         f"q = 1 mod 2^{logN + 1}), batch of {B} operands "
                         f"({4 * words_per_op * B / 2**30:.2f} GiB, HBM-resident), batched launch",
             "logN": logN, "limbs": limbs, "batch": B,
+            "engine": engine_used,
             "phase_config": cfg.astuple(),
             "chunk_operands": tuning["batch"]["chunk"] if tuning else 0,
             "parallelism": "single GPU",

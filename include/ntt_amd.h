@@ -190,6 +190,9 @@ int ntt_plan_destroy(ntt_plan* plan);
 int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out);
 /* Selects the kernel family (ntt_engine); NTT_ERR_CONFIG if unsupported for logN. */
 int ntt_plan_set_engine(ntt_plan* plan, int engine);
+/* Workgroup shape of the fused engine: 7 = 512 threads x 128 words each,
+ * 6 = 1024 threads x 64 words, 0 = the tuned choice (default 7). */
+int ntt_plan_set_fused_shape(ntt_plan* plan, int eb);
 /* Engine a launch of `count` operands would use (AUTO resolved). */
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count);
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream);
@@ -222,7 +225,9 @@ typedef struct {
     double default_fwd_ms, default_inv_ms;   /* same with ntt_default_transform_config */
     int candidates;             /* configurations timed */
     int engine;                 /* ntt_engine chosen (TWO_PASS or FUSED) */
-    double fused_fwd_ms, fused_inv_ms;       /* fused engine times (0 if not applicable) */
+    double fused_fwd_ms, fused_inv_ms;       /* best fused engine times (0 if not applicable) */
+    int fused_stagger;          /* start stagger chosen for the fused engine */
+    int fused_eb;               /* fused engine shape: 7 = 512 threads x 128 words, 6 = 1024 x 64 */
 } ntt_tune_result;
 
 int ntt_autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs,

@@ -114,13 +114,15 @@ class TuneResult(ctypes.Structure):
     _fields_ = [("cfg", TransformConfig), ("chunk", ctypes.c_size_t), ("fwd_ms", ctypes.c_double),
                 ("inv_ms", ctypes.c_double), ("default_fwd_ms", ctypes.c_double),
                 ("default_inv_ms", ctypes.c_double), ("candidates", ctypes.c_int), ("engine", ctypes.c_int),
-                ("fused_fwd_ms", ctypes.c_double), ("fused_inv_ms", ctypes.c_double)]
+                ("fused_fwd_ms", ctypes.c_double), ("fused_inv_ms", ctypes.c_double), ("fused_stagger", ctypes.c_int),
+                ("fused_eb", ctypes.c_int)]
 
     def as_dict(self):
         return {"cfg": self.cfg.astuple(), "chunk": int(self.chunk), "fwd_ms": self.fwd_ms, "inv_ms": self.inv_ms,
                 "default_fwd_ms": self.default_fwd_ms, "default_inv_ms": self.default_inv_ms,
                 "candidates": self.candidates, "engine": {1: "two_pass", 2: "fused"}.get(self.engine, self.engine),
-                "fused_fwd_ms": self.fused_fwd_ms, "fused_inv_ms": self.fused_inv_ms}
+                "fused_fwd_ms": self.fused_fwd_ms, "fused_inv_ms": self.fused_inv_ms,
+                "fused_stagger": self.fused_stagger, "fused_eb": self.fused_eb}
 
 
 @dataclass
@@ -174,6 +176,7 @@ _PROTOS = {
     "ntt_plan_inverse_many": (_I, [_P, _P, _SZ, _P]),
     "ntt_plan_run_pass": (_I, [_P, _P, _SZ, _I, _I, _P]),
     "ntt_plan_set_engine": (_I, [_P, _I]),
+    "ntt_plan_set_fused_shape": (_I, [_P, _I]),
     "ntt_plan_get_engine": (_I, [_P, _SZ]),
     "ntt_autotune": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _SZ, _I, _I, ctypes.POINTER(TuneResult)]),
     "ntt_context_set_tuning": (_I, [_P, _I, _SZ, _PCFG, _SZ, _I]),
@@ -443,6 +446,10 @@ class Plan:
 
     def set_engine(self, engine: int):
         _check(_lib.ntt_plan_set_engine(self.handle, engine), "ntt_plan_set_engine")
+
+    def set_fused_shape(self, eb: int):
+        """Fused workgroup shape: 7 (512 threads x 128 words), 6 (1024 x 64), 0 = tuned."""
+        _check(_lib.ntt_plan_set_fused_shape(self.handle, eb), "ntt_plan_set_fused_shape")
 
     def engine(self, count: int = 1) -> int:
         return int(_lib.ntt_plan_get_engine(self.handle, count))

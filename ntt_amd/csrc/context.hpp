@@ -29,7 +29,8 @@ struct LimbConst;
 struct TunedEntry {
     ntt_transform_config cfg;
     int engine = NTT_ENGINE_TWO_PASS;   // NTT_ENGINE_TWO_PASS or NTT_ENGINE_FUSED
-    bool fused_canon = false;           // fused engine: canonical arithmetic measured faster
+    int fused_eb = 7;                   // fused engine: window width (6 or 7)
+    int stagger = 0;                    // fused engine: start stagger (FusedParams::stagger)
     size_t chunk = 0;          // operands per chunk, 0 = whole launch
     double fwd_ms = 0, inv_ms = 0;
 };
@@ -53,6 +54,7 @@ struct ntt_plan {
     bool cfg_fixed = false;                        // config given by the caller
     ntt_transform_config cfg{};
     int engine = NTT_ENGINE_AUTO;
+    int fused_eb = 0;                              // fused workgroup shape: 6, 7, or 0 = tuned/default
     bool lazy = false;                             // every q < 2^30: lazy-reduction kernels
     size_t chunk = 0;
     void* d_const = nullptr;                       // one allocation holding the arrays below
@@ -77,7 +79,7 @@ int log2_bucket(size_t v);
 // Engine a launch of `count` operands runs on (resolves NTT_ENGINE_AUTO).
 int plan_engine(const ntt_plan* P, size_t count);
 int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
-                      bool allow_lazy = true);
+                      int eb = 7, int stagger = 0);
 bool fused_supported(int logN);
 int device_cus();
 }  // namespace ntt

@@ -33,16 +33,17 @@ uint32_t fused_items(int logN, size_t limbs) {
 }
 
 
-FusedFn fused_fn_lazy(int n, bool inv);    // fused_lazy.hip
-FusedFn fused_fn_canon(int n, bool inv);   // fused_canon.hip
+FusedFn fused_fn_eb6(int n, bool inv);   // fused_eb6.hip
+FusedFn fused_fn_eb7(int n, bool inv);   // fused_eb7.hip
 
-static FusedFn fused_fn(int n, bool inv, bool lazy) { return lazy ? fused_fn_lazy(n, inv) : fused_fn_canon(n, inv); }
+static FusedFn fused_fn(int n, bool inv, int eb) { return eb == 6 ? fused_fn_eb6(n, inv) : fused_fn_eb7(n, inv); }
 
 // Launches the persistent kernel of size 2^n / direction `inv`; every job in
 // p has that size and direction.
-int launch_fused(int n, bool inv, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit) {
+int launch_fused(int n, bool inv, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit) {
     if (p.total_items == 0) return NTT_OK;
-    FusedFn fn = fused_fn(n, inv, lazy);
+    if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
+    FusedFn fn = fused_fn(n, inv, eb);
     if (!fn) return NTT_ERR_CONFIG;
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
@@ -50,7 +51,7 @@ int launch_fused(int n, bool inv, bool lazy, const FusedParams& p, hipStream_t s
     std::call_once(attr_once, [&] {
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
             for (int d = 0; d < 4 && attr_err == hipSuccess; ++d)
-                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn(m, d & 1, d >> 1)),
+                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn(m, d & 1, 6 + (d >> 1))),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     });
     if (attr_err != hipSuccess) return record_hip_error(attr_err, "hipFuncSetAttribute(fused)");
@@ -59,14 +60,14 @@ int launch_fused(int n, bool inv, bool lazy, const FusedParams& p, hipStream_t s
     if (p.total_items < grid) grid = p.total_items;
     hipError_t stale = hipGetLastError();
     if (stale != hipSuccess) record_hip_error(stale, "error pending before launch (cleared)");
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(FUSED_THREADS), lds, stream, p);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(1u << (16 - eb)), lds, stream, p);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? NTT_OK : record_hip_error(e, "fused kernel launch");
 }
 
 // `count` operands of a plan as one job (limb l uses table l % limbs).
-int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
-                      bool allow_lazy) {
+int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream, int eb,
+                      int stagger) {
     if (!fused_supported(P->logN)) return NTT_ERR_CONFIG;
     if (reinterpret_cast<uintptr_t>(A) & 15) return NTT_ERR_INVALID_ARGUMENT;
     const size_t limbs = P->limbs * count;
@@ -84,7 +85,8 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.njobs = 0;
     p.total_items = fused_items(P->logN, limbs);
     p.counter = nullptr;
-    return launch_fused(P->logN, inverse, P->lazy && allow_lazy, p, stream, 0);
+    p.stagger = stagger < 0 ? 0u : (uint32_t)stagger;
+    return launch_fused(P->logN, inverse, eb, p, stream, 0);
 }
 
 }  // namespace ntt

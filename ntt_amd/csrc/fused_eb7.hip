@@ -1,0 +1,31 @@
+// Instantiations of the fused limb kernel with window width 7
+// (512 threads x 128 words per thread); one translation unit per
+// shape, so the two compile in parallel.
+#include <hip/hip_runtime.h>
+
+#include "fused_kernel.hpp"
+
+namespace ntt {
+
+FusedFn fused_fn_eb7(int n, bool inv) {
+    constexpr int EB = 7;
+    switch (n * 2 + (inv ? 1 : 0)) {
+        case 20: return fused_kernel<10, false, false, EB>;
+        case 21: return fused_kernel<10, true, false, EB>;
+        case 22: return fused_kernel<11, false, false, EB>;
+        case 23: return fused_kernel<11, true, false, EB>;
+        case 24: return fused_kernel<12, false, false, EB>;
+        case 25: return fused_kernel<12, true, false, EB>;
+        case 26: return fused_kernel<13, false, false, EB>;
+        case 27: return fused_kernel<13, true, false, EB>;
+        case 28: return fused_kernel<14, false, false, EB>;
+        case 29: return fused_kernel<14, true, false, EB>;
+        case 30: return fused_kernel<15, false, false, EB>;
+        case 31: return fused_kernel<15, true, false, EB>;
+        case 32: return fused_kernel<16, false, false, EB>;
+        case 33: return fused_kernel<16, true, false, EB>;
+        default: return nullptr;
+    }
+}
+
+}  // namespace ntt

@@ -1,8 +1,10 @@
 // Fused limb-per-workgroup negacyclic NTT / INTT for gfx950.
 //
-// One 1024-thread workgroup (16 wave64s, one per CU: it takes 132 KiB of the
-// CU's 160 KiB LDS) holds 64K words in registers: 2^(16-n) limbs of N = 2^n
-// words, 2^(n-6) threads x 64 words per limb.  Every stage of the transform
+// One workgroup (one per CU: it takes 132 KiB of the CU's 160 KiB LDS) holds
+// 64K words in registers: 2^(16-n) limbs of N = 2^n words.  With window width
+// EB, a thread holds 2^EB words and the workgroup has 2^(16-EB) threads:
+// EB = 6 is 1024 threads x 64 words (128 registers per thread available),
+// EB = 7 is 512 threads x 128 words (256 registers per thread available).  Every stage of the transform
 // runs from registers; between rounds the words move through LDS in two
 // halves.  Each limb is read from and written to HBM exactly once, instead of
 // once per pass as in the two-pass kernels (src/v04_montgomery.cu:196-565 is
@@ -10,9 +12,9 @@
 //
 // Schedule (all compile-time for a given n; checked by the host model in
 // tests/fused_model.py):
-//  * round k owns a 6-bit window [lo_k, lo_k + 6) of the word index; thread
-//    u of a limb holds the 64 words
-//        i = ((u >> lo) << (lo + 6)) | (x << lo) | (u & (2^lo - 1)),  x < 64;
+//  * round k owns an EB-bit window [lo_k, lo_k + EB) of the word index;
+//    thread u of a limb holds the 2^EB words
+//        i = ((u >> lo) << (lo + EB)) | (x << lo) | (u & (2^lo - 1));
 //  * forward windows go from the top bits down, inverse from the bottom up,
 //    and consecutive windows share a bit sigma; the exchange between them
 //    moves the words with sigma = 0, then those with sigma = 1, through a
@@ -23,7 +25,7 @@
 //    exchange: in each half a thread reads back exactly the registers it has
 //    just written out;
 //  * the twiddle of the stage on window bit j of word x is
-//    tw[((2^(n-6-lo) + (u >> lo)) << (5-j)) + (x >> (j+1))].
+//    tw[((2^(n-EB-lo) + (u >> lo)) << (EB-1-j)) + (x >> (j+1))].
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,9 +41,6 @@ namespace ntt {
 
 namespace fused {
 
-constexpr int EB = 6;
-constexpr int E = 64;
-constexpr int THREADS = 1024;
 constexpr uint32_t LDS_WORDS = 32768 + 1024;
 
 // Global-memory pointers: loads through them use an SGPR base plus a 32-bit
@@ -52,52 +51,54 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) u32x4 gu4;
 typedef __attribute__((address_space(1))) const u32x4 gcu4;
 
-__host__ __device__ constexpr int nrounds(int n) { return n >= 12 ? 3 : 2; }
+__host__ __device__ constexpr int nrounds(int n, int eb) { return n <= 2 * eb - 1 ? 2 : 3; }
 
-__host__ __device__ constexpr int win_lo(int n, bool inv, int k) {
-    return !inv ? (n >= 13 ? (k == 0 ? n - 6 : k == 1 ? n - 11 : 0)
-                           : n == 12 ? (k == 0 ? 6 : k == 1 ? 1 : 0) : (k == 0 ? n - 6 : 0))
-                : (n >= 13 ? (k == 0 ? 0 : k == 1 ? 5 : n - 6)
-                           : n == 12 ? (k == 0 ? 0 : k == 1 ? 5 : 6) : (k == 0 ? 0 : n - 6));
+// Window low bit of round k: forward from the top bits down, inverse from
+// the bottom up; consecutive windows overlap in at least one bit.
+__host__ __device__ constexpr int win_lo(int n, bool inv, int k, int eb) {
+    return nrounds(n, eb) == 2 ? (inv ? (k == 0 ? 0 : n - eb) : (k == 0 ? n - eb : 0))
+                               : (inv ? (k == 0 ? 0 : k == 1 ? eb - 1 : n - eb)
+                                      : (k == 0 ? n - eb : k == 1 ? n - 2 * eb + 1 : 0));
 }
 
 // Window-relative range [jlo, jhi] of the stages round k performs.
-__host__ __device__ constexpr int stage_jhi(int n, bool inv, int k) {
-    return !inv ? (k == 0 ? 5 : win_lo(n, inv, k - 1) - 1 - win_lo(n, inv, k)) : 5;
+__host__ __device__ constexpr int stage_jhi(int n, bool inv, int k, int eb) {
+    return !inv ? (k == 0 ? eb - 1 : win_lo(n, inv, k - 1, eb) - 1 - win_lo(n, inv, k, eb)) : eb - 1;
 }
-__host__ __device__ constexpr int stage_jlo(int n, bool inv, int k) {
-    return !inv ? 0 : (k == 0 ? 0 : win_lo(n, inv, k - 1) + 6 - win_lo(n, inv, k));
+__host__ __device__ constexpr int stage_jlo(int n, bool inv, int k, int eb) {
+    return !inv ? 0 : (k == 0 ? 0 : win_lo(n, inv, k - 1, eb) + eb - win_lo(n, inv, k, eb));
 }
 
-__host__ __device__ constexpr int sigma(int n, bool inv, int k) {
-    return !inv ? win_lo(n, inv, k) : win_lo(n, inv, k) + 5;
+// Split bit of the exchange after round k.
+__host__ __device__ constexpr int sigma(int n, bool inv, int k, int eb) {
+    return !inv ? win_lo(n, inv, k, eb) : win_lo(n, inv, k, eb) + eb - 1;
 }
 
 // Register bit of window bit b in round k.
-__host__ __device__ constexpr int perm_bit(int n, bool inv, int k, int b) {
-    int p[EB] = {0, 1, 2, 3, 4, 5};
+__host__ __device__ constexpr int perm_bit(int n, bool inv, int k, int b, int eb) {
+    int p[8] = {0, 1, 2, 3, 4, 5, 6, 7};
     for (int r = 1; r <= k; ++r) {
-        const int s = sigma(n, inv, r - 1);
-        const int ra = p[s - win_lo(n, inv, r - 1)];
-        const int jb = s - win_lo(n, inv, r);
-        int q[EB] = {0, 1, 2, 3, 4, 5};
+        const int s = sigma(n, inv, r - 1, eb);
+        const int ra = p[s - win_lo(n, inv, r - 1, eb)];
+        const int jb = s - win_lo(n, inv, r, eb);
+        int q[8] = {0, 1, 2, 3, 4, 5, 6, 7};
         q[jb] = ra;
         q[ra] = jb;
-        for (int i = 0; i < EB; ++i) p[i] = q[i];
+        for (int i = 0; i < eb; ++i) p[i] = q[i];
     }
     return p[b];
 }
 
-__host__ __device__ constexpr int reg_of(int n, bool inv, int k, int x) {
+__host__ __device__ constexpr int reg_of(int n, bool inv, int k, int x, int eb) {
     int r = 0;
-    for (int b = 0; b < EB; ++b)
-        if ((x >> b) & 1) r |= 1 << perm_bit(n, inv, k, b);
+    for (int b = 0; b < eb; ++b)
+        if ((x >> b) & 1) r |= 1 << perm_bit(n, inv, k, b, eb);
     return r;
 }
 
 // Register index as a constant expression.
-template <int n, bool INV, int K, int X>
-inline constexpr int REG = reg_of(n, INV, K, X);
+template <int n, bool INV, int K, int X, int EB>
+inline constexpr int REG = reg_of(n, INV, K, X, EB);
 
 // Calls f(std::integral_constant<int, I>) for I = 0 .. N-1, so that every
 // register index derived from I is a constant expression.
@@ -110,23 +111,49 @@ __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-template <int LO>
-__device__ __forceinline__ uint32_t vidx(uint32_t u, int x) {
-    return ((u >> LO) << (LO + EB)) | ((uint32_t)x << LO) | (u & ((1u << LO) - 1));
+// Word index of register x in a round with window LO, split into the part
+// that depends on the thread (ubase) and the part that depends on x only
+// (bit fields are disjoint, so the index is their sum): every access is a
+// per-thread base plus a compile-time offset.
+// A copy of v the compiler cannot see through: values derived from it are
+// computed where they are used instead of once, early, and kept live (the
+// kernel has no registers to spare while it holds 2^EB words per thread).
+__device__ __forceinline__ uint32_t opaque(uint32_t v) {
+    asm volatile("" : "+v"(v));
+    return v;
 }
+
+template <int LO, int EB>
+__device__ __forceinline__ uint32_t ubase(uint32_t u) {
+    return ((u >> LO) << (LO + EB)) + (u & ((1u << LO) - 1));
+}
+
+// LDS word of word i in an exchange split on bit S: i without bit S
+// ("compress"), plus one padding word per 32.  Both steps distribute over
+// sums of disjoint bit fields, so the position of a word is
+// pad(compress(thread part)) + pad(compress(register part)).
+__host__ __device__ constexpr uint32_t lds_pad(uint32_t c) { return c + (c >> 5); }
 
 template <int S>
-__device__ __forceinline__ uint32_t lds_index(uint32_t i) {
-    const uint32_t c = ((i >> (S + 1)) << S) | (i & ((1u << S) - 1));
-    return c + (c >> 5);
+__host__ __device__ constexpr uint32_t compress(uint32_t i) {
+    return ((i >> (S + 1)) << S) | (i & ((1u << S) - 1));
 }
 
-template <int n, bool INV, bool LAZY, int K>
-__device__ __forceinline__ void stages(uint32_t (&v)[E], gcu32* __restrict__ tw, uint32_t u, uint32_t q,
+template <int S, int LO, int EB>
+__device__ __forceinline__ uint32_t lds_ubase(uint32_t u) {
+    return lds_pad(compress<S>(ubase<LO, EB>(u)));
+}
+
+template <int S, int LO, int X>
+inline constexpr uint32_t LDS_OFF = lds_pad(compress<S>((uint32_t)X << LO));
+
+template <int n, bool INV, bool LAZY, int K, int EB>
+__device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict__ tw, uint32_t u, uint32_t q,
                                        uint32_t q_inv) {
-    constexpr int LO = win_lo(n, INV, K);
-    constexpr int JLO = stage_jlo(n, INV, K), JHI = stage_jhi(n, INV, K);
-    const uint32_t TB = (1u << (n - EB - LO)) + (u >> LO);
+    constexpr int E = 1 << EB;
+    constexpr int LO = win_lo(n, INV, K, EB);
+    constexpr int JLO = stage_jlo(n, INV, K, EB), JHI = stage_jhi(n, INV, K, EB);
+    const uint32_t TB = (1u << (n - EB - LO)) + (opaque(u) >> LO);
     static_for<EB>([&](auto JJ) __attribute__((always_inline)) {
         constexpr int j = INV ? decltype(JJ)::value : EB - 1 - decltype(JJ)::value;
         if constexpr (j >= JLO && j <= JHI) {
@@ -134,8 +161,8 @@ __device__ __forceinline__ void stages(uint32_t (&v)[E], gcu32* __restrict__ tw,
                 // P enumerates the words with bit j clear: x = P with a 0 inserted at bit j
                 constexpr int p = decltype(P)::value;
                 constexpr int x = ((p >> j) << (j + 1)) | (p & ((1 << j) - 1));
-                constexpr int a = REG<n, INV, K, x>, b = REG<n, INV, K, x | (1 << j)>;
-                const uint32_t w = tw[(TB << (EB - 1 - j)) + (x >> (j + 1))];
+                constexpr int a = REG<n, INV, K, x, EB>, b = REG<n, INV, K, x | (1 << j), EB>;
+                const uint32_t w = tw[(TB << (EB - 1 - j)) + (uint32_t)(x >> (j + 1))];
                 if (LAZY) {
                     if (INV)
                         gs_butterfly_lazy(v[a], v[b], w, q, q_inv);
@@ -151,43 +178,47 @@ __device__ __forceinline__ void stages(uint32_t (&v)[E], gcu32* __restrict__ tw,
     });
 }
 
-template <int n, bool INV, int K>
-__device__ __forceinline__ void exchange(uint32_t (&v)[E], uint32_t* L, uint32_t u, bool valid) {
-    constexpr int S = sigma(n, INV, K);
-    constexpr int LOA = win_lo(n, INV, K), LOB = win_lo(n, INV, K + 1);
+template <int n, bool INV, int K, int EB>
+__device__ __forceinline__ void exchange(uint32_t (&v)[1 << EB], uint32_t* L, uint32_t u, bool valid) {
+    constexpr int E = 1 << EB;
+    constexpr int S = sigma(n, INV, K, EB);
+    constexpr int LOA = win_lo(n, INV, K, EB), LOB = win_lo(n, INV, K + 1, EB);
     constexpr int JA = S - LOA, JB = S - LOB;
+    uint32_t* const LA = L + lds_ubase<S, LOA, EB>(opaque(u));
+    uint32_t* const LB = L + lds_ubase<S, LOB, EB>(opaque(u));
     static_for<2>([&](auto H) __attribute__((always_inline)) {
         constexpr int h = decltype(H)::value;
         if (valid) {
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
-                if constexpr (((x >> JA) & 1) == h) L[lds_index<S>(vidx<LOA>(u, x))] = v[REG<n, INV, K, x>];
+                if constexpr (((x >> JA) & 1) == h) LA[LDS_OFF<S, LOA, x>] = v[REG<n, INV, K, x, EB>];
             });
         }
         __syncthreads();
         if (valid) {
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
-                if constexpr (((x >> JB) & 1) == h) v[REG<n, INV, K + 1, x>] = L[lds_index<S>(vidx<LOB>(u, x))];
+                if constexpr (((x >> JB) & 1) == h) v[REG<n, INV, K + 1, x, EB>] = LB[LDS_OFF<S, LOB, x>];
             });
         }
         __syncthreads();
     });
 }
 
-template <int n, bool INV, bool LAZY>
+template <int n, bool INV, bool LAZY, int EB>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds) {
+    constexpr int E = 1 << EB;
     constexpr int G_BITS = n - EB;
     constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 6));
-    constexpr int NR = nrounds(n);
-    constexpr int LO0 = win_lo(n, INV, 0), LOL = win_lo(n, INV, NR - 1);
+    constexpr int NR = nrounds(n, EB);
+    constexpr int LO0 = win_lo(n, INV, 0, EB), LOL = win_lo(n, INV, NR - 1, EB);
     const uint32_t t = threadIdx.x;
     uint32_t slot = t >> G_BITS;
     uint32_t u = t & ((1u << G_BITS) - 1);
     // Opaque per item: otherwise the persistent loop lets the compiler hoist
     // every address derived from u out of the loop and keep it live.
     asm volatile("" : "+v"(u));
-    if (n >= 12) slot = __builtin_amdgcn_readfirstlane(slot);   // one limb per wave
+    if (G_BITS >= 6) slot = __builtin_amdgcn_readfirstlane(slot);   // one limb per wave
     const uint32_t limb = first_limb + slot;
     const bool valid = limb < job.limbs;
     const uint32_t lsafe = valid ? limb : first_limb;
@@ -201,19 +232,20 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     // ---- load (round 0 layout) ----
     if (valid) {
         if (LO0 == 0) {
-            gcu4* src = (gcu4*)(A + vidx<LO0>(u, 0));
+            gcu4* src = (gcu4*)(A + ubase<LO0, EB>(opaque(u)));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
                 constexpr int x4 = decltype(X4)::value;
                 const u32x4 w = src[x4];
-                v[REG<n, INV, 0, 4 * x4>] = w.x;
-                v[REG<n, INV, 0, 4 * x4 + 1>] = w.y;
-                v[REG<n, INV, 0, 4 * x4 + 2>] = w.z;
-                v[REG<n, INV, 0, 4 * x4 + 3>] = w.w;
+                v[REG<n, INV, 0, 4 * x4, EB>] = w.x;
+                v[REG<n, INV, 0, 4 * x4 + 1, EB>] = w.y;
+                v[REG<n, INV, 0, 4 * x4 + 2, EB>] = w.z;
+                v[REG<n, INV, 0, 4 * x4 + 3, EB>] = w.w;
             });
         } else {
+            gcu32* Ab = A + ubase<LO0, EB>(opaque(u));
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
-                v[REG<n, INV, 0, x>] = A[vidx<LO0>(u, x)];
+                v[REG<n, INV, 0, x, EB>] = Ab[x << LO0];
             });
         }
         if (!INV && (c.flags & 1)) {
@@ -225,12 +257,12 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         for (int x = 0; x < E; ++x) v[x] = 0;
     }
 
-    stages<n, INV, LAZY, 0>(v, tw, u, c.q, c.q_inv);
-    exchange<n, INV, 0>(v, L, u, valid);
-    stages<n, INV, LAZY, 1>(v, tw, u, c.q, c.q_inv);
+    stages<n, INV, LAZY, 0, EB>(v, tw, u, c.q, c.q_inv);
+    exchange<n, INV, 0, EB>(v, L, u, valid);
+    stages<n, INV, LAZY, 1, EB>(v, tw, u, c.q, c.q_inv);
     if constexpr (NR == 3) {
-        exchange<n, INV, 1>(v, L, u, valid);
-        stages<n, INV, LAZY, 2>(v, tw, u, c.q, c.q_inv);
+        exchange<n, INV, 1, EB>(v, L, u, valid);
+        stages<n, INV, LAZY, 2, EB>(v, tw, u, c.q, c.q_inv);
     }
 
     // ---- store (last round layout) ----
@@ -244,16 +276,17 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
             for (int x = 0; x < E; ++x) v[x] = canon_4q(v[x], c.q);
         }
         if (LOL == 0) {
-            gu4* dst = (gu4*)(A + vidx<LOL>(u, 0));
+            gu4* dst = (gu4*)(A + ubase<LOL, EB>(opaque(u)));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
                 constexpr int x4 = decltype(X4)::value;
-                dst[x4] = u32x4{v[REG<n, INV, NR - 1, 4 * x4>], v[REG<n, INV, NR - 1, 4 * x4 + 1>],
-                                v[REG<n, INV, NR - 1, 4 * x4 + 2>], v[REG<n, INV, NR - 1, 4 * x4 + 3>]};
+                dst[x4] = u32x4{v[REG<n, INV, NR - 1, 4 * x4, EB>], v[REG<n, INV, NR - 1, 4 * x4 + 1, EB>],
+                                v[REG<n, INV, NR - 1, 4 * x4 + 2, EB>], v[REG<n, INV, NR - 1, 4 * x4 + 3, EB>]};
             });
         } else {
+            gu32* Ab = A + ubase<LOL, EB>(opaque(u));
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
-                A[vidx<LOL>(u, x)] = v[REG<n, INV, NR - 1, x>];
+                Ab[x << LOL] = v[REG<n, INV, NR - 1, x, EB>];
             });
         }
     }
@@ -264,11 +297,12 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 // Persistent loop over work items of jobs of one size 2^n and one direction;
 // an item is 2^(16-n) consecutive limbs of one job.  Every thread of the
 // workgroup runs every barrier of every item.
-template <int n, bool INV, bool LAZY>
-__global__ __launch_bounds__(1024, 1) void fused_kernel(FusedParams p) {
+template <int n, bool INV, bool LAZY, int EB>
+__global__ __launch_bounds__(1 << (16 - EB), 1) void fused_kernel(FusedParams p) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_next;
     uint32_t iter = 0;
+    for (uint32_t s = 0, ns = (blockIdx.x & 3u) * p.stagger; s < ns; ++s) __builtin_amdgcn_s_sleep(127);
     for (;;) {
         uint32_t item;
         if (p.counter) {
@@ -292,7 +326,7 @@ __global__ __launch_bounds__(1024, 1) void fused_kernel(FusedParams p) {
             }
             job = p.jobs[lo];
         }
-        fused::run_item<n, INV, LAZY>(job, (item - job.item_begin) << (16 - n), lds);
+        fused::run_item<n, INV, LAZY, EB>(job, (item - job.item_begin) << (16 - n), lds);
     }
 }
 

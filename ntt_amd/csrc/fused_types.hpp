@@ -6,9 +6,10 @@
 
 namespace ntt {
 
-// Launch shape of the fused kernel: 1024 threads, 32K + 1K words of LDS.
-constexpr unsigned FUSED_THREADS = 1024;
+// LDS of the fused kernel: half of the workgroup's 64K words plus padding.
 constexpr uint32_t FUSED_LDS_WORDS = 32768 + 1024;
+// Default window width (7: 512 threads x 128 words; 6: 1024 threads x 64 words).
+constexpr int FUSED_DEFAULT_EB = 7;
 
 struct FusedParams;
 using FusedFn = void (*)(FusedParams);
@@ -39,6 +40,10 @@ struct FusedParams {
     uint32_t njobs;
     uint32_t total_items;
     uint32_t* counter;        // dynamic scheduling; null = static round-robin
+    // Start of workgroup b is delayed by (b % 4) * stagger sleeps of 64*127
+    // cycles each, so that the load/store phases of different CUs interleave
+    // with the compute phases of others instead of all hitting HBM at once.
+    uint32_t stagger;
 };
 
 }  // namespace ntt

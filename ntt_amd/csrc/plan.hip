@@ -226,9 +226,9 @@ static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse,
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
     if (plan_engine(plan, count) == NTT_ENGINE_FUSED) {
         TunedEntry t;
-        const bool canon = !plan->cfg_fixed && plan->ctx->tuned_lookup(plan->logN, plan->limbs * count, &t) &&
-                           t.fused_canon;
-        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, !canon);
+        const bool tuned = !plan->cfg_fixed && plan->ctx->tuned_lookup(plan->logN, plan->limbs * count, &t);
+        const int eb = plan->fused_eb ? plan->fused_eb : (tuned ? t.fused_eb : FUSED_DEFAULT_EB);
+        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, eb, tuned ? t.stagger : 0);
     }
     return launch_transform(plan_launch(plan, A_dev, count, inverse), (hipStream_t)stream);
 }
@@ -247,6 +247,13 @@ int ntt_plan_set_engine(ntt_plan* plan, int engine) {
         return NTT_ERR_INVALID_ARGUMENT;
     if (engine == NTT_ENGINE_FUSED && !fused_supported(plan->logN)) return NTT_ERR_CONFIG;
     plan->engine = engine;
+    return NTT_OK;
+}
+
+int ntt_plan_set_fused_shape(ntt_plan* plan, int eb) {
+    if (!plan) return NTT_ERR_INVALID_ARGUMENT;
+    if (eb != 0 && eb != 6 && eb != 7) return NTT_ERR_CONFIG;
+    plan->fused_eb = eb;
     return NTT_OK;
 }
 

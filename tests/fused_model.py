@@ -15,57 +15,56 @@ the hazard rule above, LDS ownership, and the transform against the oracle.
 """
 from __future__ import annotations
 
-E_BITS = 6
-E = 1 << E_BITS
-THREADS = 1024
-HALF_WORDS = 32768
+E_BITS = 6          # window width; the kernel is built for 6 (1024 threads) and 7 (512 threads)
+WG_WORDS = 1 << 16  # words held by one workgroup
 
 
-def windows(n: int, inv: bool):
+def nrounds(n: int, eb: int = None) -> int:
+    eb = E_BITS if eb is None else eb
+    return 2 if n <= 2 * eb - 1 else 3
+
+
+def windows(n: int, inv: bool, eb: int = None):
     """Window low bits of the rounds, in execution order."""
-    if not inv:
-        if n >= 13:
-            return [n - 6, n - 11, 0]
-        if n == 12:
-            return [6, 1, 0]
-        return [n - 6, 0]
-    if n >= 13:
-        return [0, 5, n - 6]
-    if n == 12:
-        return [0, 5, 6]
-    return [0, n - 6]
+    eb = E_BITS if eb is None else eb
+    if nrounds(n, eb) == 2:
+        return [0, n - eb] if inv else [n - eb, 0]
+    return [0, eb - 1, n - eb] if inv else [n - eb, n - 2 * eb + 1, 0]
 
 
-def stage_bits(n: int, inv: bool):
+def stage_bits(n: int, inv: bool, eb: int = None):
     """Per round, the window-relative bits whose stages run, in order."""
-    ws = windows(n, inv)
+    eb = E_BITS if eb is None else eb
+    ws = windows(n, inv, eb)
     out = []
     for k, lo in enumerate(ws):
         if not inv:
-            top = lo + 5 if k == 0 else ws[k - 1] - 1
+            top = lo + eb - 1 if k == 0 else ws[k - 1] - 1
             out.append([b - lo for b in range(top, lo - 1, -1)])
         else:
-            bot = lo if k == 0 else ws[k - 1] + 6
-            out.append([b - lo for b in range(bot, lo + 6)])
+            bot = lo if k == 0 else ws[k - 1] + eb
+            out.append([b - lo for b in range(bot, lo + eb)])
     return out
 
 
-def sigmas(n: int, inv: bool):
-    ws = windows(n, inv)
-    return [(ws[k] if not inv else ws[k] + 5) for k in range(len(ws) - 1)]
+def sigmas(n: int, inv: bool, eb: int = None):
+    eb = E_BITS if eb is None else eb
+    ws = windows(n, inv, eb)
+    return [(ws[k] if not inv else ws[k] + eb - 1) for k in range(len(ws) - 1)]
 
 
-def perms(n: int, inv: bool):
+def perms(n: int, inv: bool, eb: int = None):
     """perm[k][window bit] = register bit, per round."""
-    ws = windows(n, inv)
-    sg = sigmas(n, inv)
-    out = [list(range(E_BITS))]
+    eb = E_BITS if eb is None else eb
+    ws = windows(n, inv, eb)
+    sg = sigmas(n, inv, eb)
+    out = [list(range(eb))]
     for k in range(1, len(ws)):
         s = sg[k - 1]
         ra = out[k - 1][s - ws[k - 1]]
         jb = s - ws[k]
-        assert 0 <= jb < E_BITS, "split bit must be a register bit of both windows"
-        p = list(range(E_BITS))
+        assert 0 <= jb < eb, "split bit must be a register bit of both windows"
+        p = list(out[0])
         # transposition jb <-> ra
         p[jb], p[ra] = ra, jb
         out.append(p)
@@ -74,14 +73,15 @@ def perms(n: int, inv: bool):
 
 def reg_of(p, xw: int) -> int:
     r = 0
-    for b in range(E_BITS):
+    for b in range(len(p)):
         if (xw >> b) & 1:
             r |= 1 << p[b]
     return r
 
 
-def vidx(lo: int, u: int, x: int) -> int:
-    return ((u >> lo) << (lo + E_BITS)) | (x << lo) | (u & ((1 << lo) - 1))
+def vidx(lo: int, u: int, x: int, eb: int = None) -> int:
+    eb = E_BITS if eb is None else eb
+    return ((u >> lo) << (lo + eb)) | (x << lo) | (u & ((1 << lo) - 1))
 
 
 def compress(i: int, s: int) -> int:
@@ -93,7 +93,7 @@ def lds_slot_stride(n: int) -> int:
     return half + (half >> 5)
 
 
-def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False):
+def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False, eb: int = None):
     """limbs_data: list of 2^(16-n) limbs (lists), modified copies returned.
 
     lazy=False: canonical arithmetic on normal-domain values (scale is a plain
@@ -101,12 +101,15 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False):
     (tests/lazy_arith.py) with Montgomery-form twiddles; scale is the
     Montgomery multiplier applied on load (forward) or store (inverse)."""
     from tests import lazy_arith as la
+    eb = E_BITS if eb is None else eb
+    E = 1 << eb
+    THREADS = WG_WORDS >> eb
     N = 1 << n
-    g = n - E_BITS
+    g = n - eb
     lpw = 1 << (16 - n)
     assert len(limbs_data) <= lpw
     A = [list(a) for a in limbs_data]
-    ws, sb, pm, sg = windows(n, inv), stage_bits(n, inv), perms(n, inv), sigmas(n, inv)
+    ws, sb, pm, sg = windows(n, inv, eb), stage_bits(n, inv, eb), perms(n, inv, eb), sigmas(n, inv, eb)
     assert sum(len(s) for s in sb) == n
     stride = lds_slot_stride(n)
     lds_words = lpw * stride
@@ -121,7 +124,7 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False):
                     continue
                 slot, u = t >> g, t & ((1 << g) - 1)
                 for xw in range(E):
-                    i = vidx(lo, u, xw)
+                    i = vidx(lo, u, xw, eb)
                     assert 0 <= i < N
                     v = A[slot][i]
                     if not inv and scale is not None:
@@ -135,12 +138,12 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False):
             slot, u = t >> g, t & ((1 << g) - 1)
             q = qs[slot]
             qinv = la.qinv_of(q)
-            tb = (1 << (n - E_BITS - lo)) + (u >> lo)
+            tb = (1 << (n - eb - lo)) + (u >> lo)
             for j in sb[k]:
                 for xw in range(E):
                     if (xw >> j) & 1:
                         continue
-                    ti = (tb << (E_BITS - 1 - j)) + (xw >> (j + 1))
+                    ti = (tb << (eb - 1 - j)) + (xw >> (j + 1))
                     assert 0 < ti < N
                     w = tws[slot][ti]
                     a, b = reg_of(p, xw), reg_of(p, xw | (1 << j))
@@ -171,7 +174,7 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False):
                 for xw in range(E):
                     if ((xw >> ja) & 1) != h:
                         continue
-                    i = vidx(lo, u, xw)
+                    i = vidx(lo, u, xw, eb)
                     assert ((i >> s) & 1) == h
                     c = compress(i, s)
                     a = slot * stride + c + (c >> 5)
@@ -187,7 +190,7 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False):
                         continue
                     r = reg_of(pb, xw)
                     assert r in written[t], "register read before its value was saved"
-                    i = vidx(lo_b, u, xw)
+                    i = vidx(lo_b, u, xw, eb)
                     c = compress(i, s)
                     a = slot * stride + c + (c >> 5)
                     assert lds[a] is not None, "read of an unwritten LDS word"
@@ -200,7 +203,7 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False):
             continue
         slot, u = t >> g, t & ((1 << g) - 1)
         for xw in range(E):
-            i = vidx(lo, u, xw)
+            i = vidx(lo, u, xw, eb)
             v = regs[t][reg_of(p, xw)]
             if lazy:
                 v = la.mont(v, scale[slot], qs[slot], la.qinv_of(qs[slot])) if inv else la.canon_4q(v, qs[slot])

@@ -248,10 +248,10 @@ def test_autotune_result_is_valid_and_exact(env, tmp_path):
 
 
 @pytest.mark.parametrize("logN", [10, 11, 12, 13, 14, 15, 16])
-@pytest.mark.parametrize("engine", [2, 1])
-def test_engines_all_sizes(env, logN, engine):
-    """Fused limb kernel and two-pass kernels, bit-exact; limb counts that
-    leave the last fused work item partly empty."""
+@pytest.mark.parametrize("engine,eb", [(2, 7), (2, 6), (1, 0)])
+def test_engines_all_sizes(env, logN, engine, eb):
+    """Fused limb kernel (both workgroup shapes) and two-pass kernels,
+    bit-exact; limb counts that leave the last fused work item partly empty."""
     ntt_amd, gu, orc = env
     lpw = 1 << (16 - logN)
     for limbs in sorted({1, 3, lpw + 1}):
@@ -260,10 +260,11 @@ def test_engines_all_sizes(env, logN, engine):
             ctx = ntt_amd.Context(0)
             plan = ctx.plan(logN, moduli, domain)
             plan.set_engine(engine)
+            plan.set_fused_shape(eb)
             assert plan.engine(1) == engine
             x = gu.random_operand(logN, moduli, 1000 + limbs)
             A = gu.to_dev(x)
-            _print("case engine", engine, logN, limbs, domain)
+            _print("case engine", engine, eb, logN, limbs, domain)
             plan.forward(A)
             gu.sync()
             assert np.array_equal(gu.to_host(A).reshape(x.shape), orc.forward(x, logN, moduli, domain))
@@ -280,6 +281,7 @@ def test_fused_many_operands_period(env):
     ctx = ntt_amd.Context(0)
     plan = ctx.plan(logN, moduli, 1)
     plan.set_engine(2)
+    plan.set_fused_shape(6)
     xs = np.stack([gu.random_operand(logN, moduli, 70 + k) for k in range(count)])
     A = gu.to_dev(xs)
     _print("case fused-many")
