@@ -11,9 +11,12 @@ initial contents (the roundtrip check).
 
 Prints ONE JSON line on stdout (diagnostics go to stderr):
   value      total butterflies (forward + inverse) per second, in G/s
-  roofline   the slowest pass kernel: algorithmic bytes per launch
-             (8 B per word: read once, written once) / its mean duration,
-             measured with HIP events on the launch stream
+  roofline   the slowest kernel of the engine in use: algorithmic bytes per
+             launch (8 B per word: read once, written once) / its mean
+             duration, measured with HIP events on the launch stream; traffic
+             = counter-measured HBM bytes of that kernel (profiles/)
+  compute_roof  the same kernel's butterflies/s against the butterfly roof
+             measured live (the kernels' butterfly code on registers only)
   cpu_baseline  the oracle's butterfly loops (src/v04_montgomery.cu:742-784
              restated in C) on one host core, table generation excluded
 """
@@ -52,9 +55,30 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="use the default phase config instead of the autotuner's")
     ap.add_argument("--engine", choices=["auto", "fused", "two_pass"], default="auto",
                     help="force a kernel family for the batched leg")
+    ap.add_argument("--fused-shape", type=int, choices=[0, 6, 7], default=0,
+                    help="force the fused workgroup shape (0: the tuned one)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run one device-to-device copy of the batch (known bytes, for counter calibration)")
+                    help="also run one plain copy of the batch (known bytes, for counter calibration)")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "hbm_traffic.txt"),
+                    help="HBM bytes per dispatch measured with rocprofv3 counters (see that file)")
     return ap.parse_args()
+
+
+def hbm_traffic(path: str, kernel: str, workload: str):
+    """HBM bytes per dispatch of `kernel` on `workload` from a counter summary:
+    lines `kernel=<name>;workload=<tag>;fetch_bytes=<n>;write_bytes=<n>` (already
+    scaled by the copy-kernel calibration).  None when not measured."""
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        if ln.startswith("#") or "=" not in ln:
+            continue
+        kv = dict(f.split("=", 1) for f in ln.strip().split(";") if "=" in f)
+        if kv.get("kernel") == kernel and kv.get("workload") == workload:
+            return int(kv["fetch_bytes"]) + int(kv["write_bytes"])
+    return None
 
 
 def cpu_baseline(logN: int, moduli, seconds: float, threads: int = 1):
@@ -128,6 +152,8 @@ def main():
     plan = ctx.plan(logN, moduli, ntt_amd.MONTGOMERY)
     if args.engine != "auto":
         plan.set_engine({"fused": ntt_amd.ENGINE_FUSED, "two_pass": ntt_amd.ENGINE_TWO_PASS}[args.engine])
+    if args.fused_shape:
+        plan.set_fused_shape(args.fused_shape)
     engine_used = {1: "two_pass", 2: "fused"}.get(plan.engine(B), "?")
     cfg = ntt_amd.TransformConfig.make(*tuning["batch"]["cfg"]) if tuning else plan.config
     words_per_op = limbs * N
@@ -141,7 +167,7 @@ def main():
 
     if args.copy_calibration:
         scratch = dev.DeviceBuffer(4 * words_per_op * B)
-        scratch.copy_from(pool, stream=stream)
+        dev.stream_copy(scratch, pool, 4 * words_per_op * B, stream)
         stream.synchronize()
         log(f"copy calibration: {4 * words_per_op * B} bytes read and written")
         del scratch
@@ -177,27 +203,53 @@ def main():
     if not ok:
         log("ROUNDTRIP MISMATCH after the timed region")
 
-    # ---- per-pass kernel durations (events on the launch stream) ----
-    names = ["ntt_pass0_column", "ntt_pass1_row", "intt_pass0_row", "intt_pass1_column"]
-    ev = [dev.Event() for _ in range(5)]
+    # ---- kernel durations of the engine in use (HIP events on the launch stream) ----
+    eb = plan.fused_shape(B)
+    lazy = all(q < (1 << 30) for q, _ in moduli)   # two-pass kernels then use lazy reduction
+    if engine_used == "fused":
+        kern = {f"fused_kernel<{logN}, false, false, {eb}>": lambda: plan.forward(pool, B, stream),
+                f"fused_kernel<{logN}, true, false, {eb}>": lambda: plan.inverse(pool, B, stream)}
+    else:
+        mode = 2 if lazy else 0
+        c = cfg.astuple()
+        kern = {f"phase_kernel<{c[0][1]}, false, true, {mode}>": lambda: plan.run_pass(pool, B, 0, 0, stream),
+                f"phase_kernel<{c[1][1]}, false, false, {mode}>": lambda: plan.run_pass(pool, B, 0, 1, stream),
+                f"phase_kernel<{c[2][1]}, true, false, {mode}>": lambda: plan.run_pass(pool, B, 1, 0, stream),
+                f"phase_kernel<{c[3][1]}, true, true, {mode}>": lambda: plan.run_pass(pool, B, 1, 1, stream)}
+    names = list(kern)
+    ev = [dev.Event() for _ in range(len(names) + 1)]
     per = {n: [] for n in names}
     for _ in range(max(5, args.steps // 2)):
         ev[0].record(stream)
-        plan.run_pass(pool, B, 0, 0, stream); ev[1].record(stream)
-        plan.run_pass(pool, B, 0, 1, stream); ev[2].record(stream)
-        plan.run_pass(pool, B, 1, 0, stream); ev[3].record(stream)
-        plan.run_pass(pool, B, 1, 1, stream); ev[4].record(stream)
-        ev[4].synchronize()
+        for i, n in enumerate(names):
+            kern[n]()
+            ev[i + 1].record(stream)
+        ev[-1].synchronize()
         for i, n in enumerate(names):
             per[n].append(ev[i].elapsed_ms(ev[i + 1]))
-    pass_ms = {n: statistics.mean(v) for n, v in per.items()}
-    dom = max(pass_ms, key=pass_ms.get)
-    achieved = B * bytes_per_transform / (pass_ms[dom] * 1e-3) / 1e9
+    kernel_ms = {n: statistics.mean(v) for n, v in per.items()}
+    dom = max(kernel_ms, key=kernel_ms.get)
+    bytes_per_launch = B * bytes_per_transform
+    achieved = bytes_per_launch / (kernel_ms[dom] * 1e-3) / 1e9
+    workload_tag = f"logN{logN}x{limbs}_batch{B}"
+    traffic = hbm_traffic(args.traffic, dom, workload_tag)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "kernel_ms": round(pass_ms[dom], 4),
-                "bytes_per_launch": B * bytes_per_transform}
-    log("pass ms: " + ", ".join(f"{n} {v:.4f}" for n, v in pass_ms.items()))
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
+                "kernel_ms": round(kernel_ms[dom], 4), "bytes_per_launch": bytes_per_launch,
+                "units_per_launch": f"{B * limbs} limbs of 2^{logN} words, 8 B per word (read once, written once)"}
+    log("kernel ms: " + ", ".join(f"{n} {v:.4f}" for n, v in kernel_ms.items()))
+
+    # the binding compute roof: the same butterfly code on register-resident operands
+    inverse_dom = ", true," in dom
+    variant = (dev.BUTTERFLY_GS if inverse_dom else dev.BUTTERFLY_CT) + (1 if (lazy and engine_used != "fused")
+                                                                          else 0)
+    roof_bfly = dev.butterfly_roof(variant, stream)
+    bfly_dom = B * bfly_per_transform / (kernel_ms[dom] * 1e-3)
+    compute_roof = {"bound": "valu", "unit": "Gbutterfly/s", "achieved": round(bfly_dom / 1e9, 1),
+                    "peak": round(roof_bfly / 1e9, 1), "frac": round(bfly_dom / roof_bfly, 4),
+                    "butterfly": ("GS" if inverse_dom else "CT") + (" lazy" if variant & 1 else " canonical"),
+                    "kernel": dom}
+    log(f"compute roof: {compute_roof}")
 
     extras = {}
     if not args.no_extras:
@@ -275,6 +327,7 @@ def main():
                         f"({4 * words_per_op * B / 2**30:.2f} GiB, HBM-resident), batched launch",
             "logN": logN, "limbs": limbs, "batch": B,
             "engine": engine_used,
+            "fused_shape": eb if engine_used == "fused" else None,
             "phase_config": cfg.astuple(),
             "chunk_operands": tuning["batch"]["chunk"] if tuning else 0,
             "parallelism": "single GPU",
@@ -283,7 +336,8 @@ def main():
         "roundtrip_ok": ok,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "pass_ms": {k: round(v, 4) for k, v in pass_ms.items()},
+        "compute_roof": compute_roof,
+        "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "autotune": tuning or None,
         **extras,
     }

@@ -193,6 +193,8 @@ int ntt_plan_set_engine(ntt_plan* plan, int engine);
 /* Workgroup shape of the fused engine: 7 = 512 threads x 128 words each,
  * 6 = 1024 threads x 64 words, 0 = the tuned choice (default 7). */
 int ntt_plan_set_fused_shape(ntt_plan* plan, int eb);
+/* The fused shape a run of `count` operands would use (6 or 7). */
+int ntt_plan_get_fused_shape(const ntt_plan* plan, size_t count);
 /* Engine a launch of `count` operands would use (AUTO resolved). */
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count);
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream);
@@ -237,6 +239,47 @@ int ntt_context_set_tuning(ntt_context* ctx, int logN, size_t total_limbs,
 int ntt_context_clear_tuning(ntt_context* ctx);
 int ntt_context_save_tuning(const ntt_context* ctx, const char* path);
 int ntt_context_load_tuning(ntt_context* ctx, const char* path);
+
+/* ---------------------------------------------------------------------------
+ * Batches: many independent transforms of uneven size, limb count, modulus
+ * set and direction, launched together.  No counterpart in the reference,
+ * whose main() runs one fixed-size transform at a time
+ * (src/v04_montgomery.cu:969-1062).  Jobs of one (logN, direction) with
+ * 10 <= logN <= 16 run as ONE persistent fused kernel that walks a
+ * device-resident job table; other sizes run as two-pass launches per job.
+ * Groups are spread over forked streams so they overlap on the device.  The
+ * whole launch sequence can be captured once into a hipGraph and replayed.
+ * Twiddle tables come from the context's cache (shared across jobs).
+ * ------------------------------------------------------------------------- */
+typedef struct ntt_batch ntt_batch;
+
+typedef struct {
+    uint32_t* data;                       /* device operand, limbs x 2^logN words, in place */
+    const ntt_modulus_config* moduli;     /* host array of `limbs` moduli (read at create) */
+    uint32_t limbs;
+    int32_t logN;
+    int32_t direction;                    /* ntt_direction */
+    int32_t domain;                       /* ntt_domain */
+} ntt_batch_job;
+
+/* Validates every job and builds (or reuses) per-modulus-set constants. */
+int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, ntt_batch** out);
+int ntt_batch_destroy(ntt_batch* batch);
+/* Enqueues every job on `stream` (and on the batch's own streams, joined back
+ * into `stream`); asynchronous. */
+int ntt_batch_run(ntt_batch* batch, void* stream);
+/* Captures ntt_batch_run into a hipGraph (replacing an earlier capture). */
+int ntt_batch_capture(ntt_batch* batch, void* stream);
+/* Launches the captured graph on `stream` (captures first if needed). */
+int ntt_batch_replay(ntt_batch* batch, void* stream);
+/* Points job `index` at another operand of the same shape.  Fused-group jobs
+ * read their operand address from the device job table, so a captured graph
+ * stays valid; for a two-pass job the graph is dropped and re-captured by the
+ * next replay.  Synchronous with respect to the device. */
+int ntt_batch_set_data(ntt_batch* batch, size_t index, uint32_t* data);
+/* fused_groups: persistent fused launches; two_pass_jobs: jobs on two-pass
+ * kernels; streams: concurrent streams used. */
+int ntt_batch_info(const ntt_batch* batch, int* fused_groups, int* two_pass_jobs, int* streams);
 
 #ifdef __cplusplus
 }

@@ -54,6 +54,16 @@ int ntt_rt_event_elapsed_ms(float* ms, void* start, void* stop);
  * large synthetic operands on the device without a host copy. */
 int ntt_rt_fill_uniform(uint32_t* dst, const uint32_t* moduli_dev, size_t limbs, int logN,
                         size_t count, uint64_t seed, void* stream);
+/* Butterfly roof: butterflies per second the whole device sustains running
+ * the transform kernels' own butterfly code on register-resident operands
+ * (no memory or LDS traffic).  variant 0: Cooley-Tukey, canonical; 1: CT,
+ * lazy (q < 2^30); 2: Gentleman-Sande, canonical; 3: GS, lazy.  Blocks until
+ * done (about a millisecond). */
+int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream);
+/* Plain streaming copy of `bytes` (multiple of 16, 16-byte aligned): reads and
+ * writes each byte once; the reference kernel for bandwidth and for scaling
+ * the HBM byte counters. */
+int ntt_rt_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
 
 #ifdef __cplusplus
 }

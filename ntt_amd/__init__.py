@@ -125,6 +125,12 @@ class TuneResult(ctypes.Structure):
                 "fused_stagger": self.fused_stagger, "fused_eb": self.fused_eb}
 
 
+class BatchJob(ctypes.Structure):
+    """ntt_batch_job of include/ntt_amd.h."""
+    _fields_ = [("data", ctypes.c_void_p), ("moduli", ctypes.c_void_p), ("limbs", ctypes.c_uint32),
+                ("logN", ctypes.c_int32), ("direction", ctypes.c_int32), ("domain", ctypes.c_int32)]
+
+
 @dataclass
 class VersionConfig:
     """VersionConfig{logN, moduli, transform}, src/ntt_config.h:25-29."""
@@ -177,12 +183,20 @@ _PROTOS = {
     "ntt_plan_run_pass": (_I, [_P, _P, _SZ, _I, _I, _P]),
     "ntt_plan_set_engine": (_I, [_P, _I]),
     "ntt_plan_set_fused_shape": (_I, [_P, _I]),
+    "ntt_plan_get_fused_shape": (_I, [_P, _SZ]),
     "ntt_plan_get_engine": (_I, [_P, _SZ]),
     "ntt_autotune": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _SZ, _I, _I, ctypes.POINTER(TuneResult)]),
     "ntt_context_set_tuning": (_I, [_P, _I, _SZ, _PCFG, _SZ, _I]),
     "ntt_context_clear_tuning": (_I, [_P]),
     "ntt_context_save_tuning": (_I, [_P, ctypes.c_char_p]),
     "ntt_context_load_tuning": (_I, [_P, ctypes.c_char_p]),
+    "ntt_batch_create": (_I, [_P, _P, _SZ, ctypes.POINTER(_P)]),
+    "ntt_batch_destroy": (_I, [_P]),
+    "ntt_batch_run": (_I, [_P, _P]),
+    "ntt_batch_capture": (_I, [_P, _P]),
+    "ntt_batch_replay": (_I, [_P, _P]),
+    "ntt_batch_set_data": (_I, [_P, _SZ, _P]),
+    "ntt_batch_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
 }
 
 
@@ -420,6 +434,56 @@ class Context:
     def load_tuning(self, path: str):
         _check(_lib.ntt_context_load_tuning(self.handle, path.encode()), "ntt_context_load_tuning")
 
+    def batch(self, jobs) -> "Batch":
+        return Batch(self, jobs)
+
+
+class Batch:
+    """Uneven transforms launched together (ntt_batch_*): jobs are
+    (device operand, logN, moduli, direction, domain) tuples."""
+
+    def __init__(self, ctx: Context, jobs):
+        self.ctx = ctx
+        keep = []
+        arr = (BatchJob * len(jobs))()
+        for k, (data, logN, moduli, direction, domain) in enumerate(jobs):
+            ms = as_moduli(moduli)
+            m = (ModulusConfig * len(ms))(*ms)
+            keep.append(m)
+            arr[k] = BatchJob(_addr(data), ctypes.addressof(m), len(ms), logN, direction, domain)
+        h = ctypes.c_void_p()
+        _check(_lib.ntt_batch_create(ctx.handle, arr, len(jobs), ctypes.byref(h)), "ntt_batch_create")
+        self.handle = h
+        self.njobs = len(jobs)
+
+    def run(self, stream=None):
+        _check(_lib.ntt_batch_run(self.handle, _stream(stream)), "ntt_batch_run")
+
+    def capture(self, stream=None):
+        _check(_lib.ntt_batch_capture(self.handle, _stream(stream)), "ntt_batch_capture")
+
+    def replay(self, stream=None):
+        _check(_lib.ntt_batch_replay(self.handle, _stream(stream)), "ntt_batch_replay")
+
+    def set_data(self, index: int, data):
+        _check(_lib.ntt_batch_set_data(self.handle, index, _addr(data)), "ntt_batch_set_data")
+
+    def info(self) -> dict:
+        a, b, c = _I(0), _I(0), _I(0)
+        _check(_lib.ntt_batch_info(self.handle, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "ntt_batch_info")
+        return {"fused_groups": a.value, "two_pass_jobs": b.value, "streams": c.value}
+
+    def close(self):
+        if self.handle:
+            _lib.ntt_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
 
 class Plan:
     def __init__(self, ctx: Context, logN: int, moduli, domain: int = MONTGOMERY,
@@ -446,6 +510,10 @@ class Plan:
 
     def set_engine(self, engine: int):
         _check(_lib.ntt_plan_set_engine(self.handle, engine), "ntt_plan_set_engine")
+
+    def fused_shape(self, count: int = 1) -> int:
+        """Fused workgroup shape a run of `count` operands uses."""
+        return _lib.ntt_plan_get_fused_shape(self.handle, count)
 
     def set_fused_shape(self, eb: int):
         """Fused workgroup shape: 7 (512 threads x 128 words), 6 (1024 x 64), 0 = tuned."""
@@ -479,3 +547,6 @@ def butterflies(logN: int, limbs: int) -> float:
 def algorithmic_bytes(logN: int, limbs: int) -> int:
     """Each uint32 word read once and written once."""
     return 8 * (1 << logN) * limbs
+
+
+from . import workloads  # noqa: E402,F401

@@ -1,0 +1,313 @@
+// Batches of uneven transforms (ntt_batch_* in include/ntt_amd.h).
+//
+// Jobs are grouped by (logN, direction).  A group with a fused-capable size
+// becomes one persistent fused launch over a device-resident table of
+// FusedJob descriptors (one per job, each with its own limb count, table
+// pointers and per-limb constants), so any number of jobs of that size costs
+// one kernel.  Jobs of other sizes run as two-pass launches.  Work units are
+// balanced over the caller's stream plus up to three streams owned by the
+// batch (fork/join through events), so groups overlap on the device.  The
+// same enqueue sequence is captured into a hipGraph for replay; it contains
+// kernel launches and event edges only (no memset or copy nodes).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ntt_amd.h"
+#include "context.hpp"
+#include "fused_types.hpp"
+#include "host_math.hpp"
+#include "launch.hpp"
+
+namespace ntt {
+int launch_fused(int n, bool inv, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
+int fused_prepare();
+uint32_t fused_items(int logN, size_t limbs);
+}  // namespace ntt
+
+using namespace ntt;
+
+namespace {
+
+constexpr int BATCH_AUX_STREAMS = 3;
+
+struct PlanDeleter {
+    void operator()(ntt_plan* p) const { ntt_plan_destroy(p); }
+};
+
+struct FusedGroup {
+    int n = 0;
+    bool inverse = false;
+    int eb = FUSED_DEFAULT_EB;
+    size_t desc_begin = 0;          // first descriptor in the batch's table
+    size_t njobs = 0;
+    uint32_t items = 0;
+    double work = 0;                // butterflies, for stream balancing
+};
+
+struct Unit {                       // one enqueue: a fused group or a two-pass job
+    int group = -1;
+    size_t job = 0;
+    double work = 0;
+};
+
+}  // namespace
+
+struct ntt_batch {
+    ntt_context* ctx = nullptr;
+    std::vector<ntt_batch_job> jobs;            // moduli pointer cleared after create
+    std::vector<ntt_plan*> job_plan;
+    std::vector<int> job_group;                 // fused group index or -1
+    std::vector<size_t> job_desc;               // descriptor index for fused jobs
+    std::map<std::tuple<int, int, std::vector<uint32_t>>, std::unique_ptr<ntt_plan, PlanDeleter>> plans;
+    std::vector<FusedGroup> groups;
+    std::vector<FusedJob> desc;                 // host copy of the device job table
+    FusedJob* d_desc = nullptr;
+    std::vector<std::vector<Unit>> lanes;       // lane 0 = caller's stream
+    hipStream_t aux[BATCH_AUX_STREAMS] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[BATCH_AUX_STREAMS] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~ntt_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto& s : aux)
+            if (s) (void)hipStreamDestroy(s);
+        for (auto& e : join)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        if (d_desc) (void)hipFree(d_desc);
+    }
+};
+
+namespace {
+
+int ck(hipError_t e, const char* where) { return e == hipSuccess ? NTT_OK : record_hip_error(e, where); }
+
+int enqueue(ntt_batch* b, hipStream_t stream) {
+    const size_t nl = b->lanes.size();
+    int rc = NTT_OK;
+    if (nl > 1) {
+        if ((rc = ck(hipEventRecord(b->fork, stream), "hipEventRecord(fork)"))) return rc;
+        for (size_t l = 1; l < nl; ++l)
+            if ((rc = ck(hipStreamWaitEvent(b->aux[l - 1], b->fork, 0), "hipStreamWaitEvent(fork)"))) return rc;
+    }
+    for (size_t l = 0; l < nl && !rc; ++l) {
+        hipStream_t s = l == 0 ? stream : b->aux[l - 1];
+        for (const Unit& u : b->lanes[l]) {
+            if (u.group >= 0) {
+                const FusedGroup& g = b->groups[(size_t)u.group];
+                FusedParams p{};
+                p.jobs = b->d_desc + g.desc_begin;
+                p.njobs = (uint32_t)g.njobs;
+                p.total_items = g.items;
+                p.counter = nullptr;          // static round-robin: nothing to reset between replays
+                p.stagger = 0;
+                rc = launch_fused(g.n, g.inverse, g.eb, p, s, 0);
+            } else {
+                const ntt_batch_job& j = b->jobs[u.job];
+                rc = launch_transform(plan_launch(b->job_plan[u.job], j.data, 1, j.direction == NTT_INVERSE), s);
+            }
+            if (rc) break;
+        }
+    }
+    if (nl > 1) {
+        // join even after an error so no stream is left forked from a capture
+        for (size_t l = 1; l < nl; ++l) {
+            int r1 = ck(hipEventRecord(b->join[l - 1], b->aux[l - 1]), "hipEventRecord(join)");
+            int r2 = ck(hipStreamWaitEvent(stream, b->join[l - 1], 0), "hipStreamWaitEvent(join)");
+            if (!rc) rc = r1 ? r1 : r2;
+        }
+    }
+    return rc;
+}
+
+void drop_graph(ntt_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, ntt_batch** out) {
+    if (!ctx || !jobs || !out || njobs == 0) return NTT_ERR_INVALID_ARGUMENT;
+    *out = nullptr;
+    std::unique_ptr<ntt_batch> b(new ntt_batch());
+    b->ctx = ctx;
+    b->jobs.assign(jobs, jobs + njobs);
+    b->job_plan.assign(njobs, nullptr);
+    b->job_group.assign(njobs, -1);
+    b->job_desc.assign(njobs, 0);
+    int rc = NTT_OK;
+    std::map<std::pair<int, int>, std::vector<size_t>> by_shape;   // (logN, inverse) -> fused jobs
+    std::vector<size_t> two_pass;
+    for (size_t i = 0; i < njobs; ++i) {
+        const ntt_batch_job& j = jobs[i];
+        if (!j.data || !j.moduli || j.limbs == 0 || j.logN < 1 || j.logN > NTT_MAX_LOG_N) return NTT_ERR_INVALID_ARGUMENT;
+        if (j.direction != NTT_FORWARD && j.direction != NTT_INVERSE) return NTT_ERR_INVALID_ARGUMENT;
+        if (j.domain != NTT_DOMAIN_PLAIN && j.domain != NTT_DOMAIN_MONTGOMERY) return NTT_ERR_INVALID_ARGUMENT;
+        if (reinterpret_cast<uintptr_t>(j.data) & 15) return NTT_ERR_INVALID_ARGUMENT;
+        std::vector<uint32_t> key_q(2 * (size_t)j.limbs);
+        for (uint32_t l = 0; l < j.limbs; ++l) {
+            key_q[2 * l] = j.moduli[l].qi;
+            key_q[2 * l + 1] = j.moduli[l].primitive_root;
+        }
+        auto key = std::make_tuple(j.logN, j.domain, std::move(key_q));
+        auto it = b->plans.find(key);
+        if (it == b->plans.end()) {
+            ntt_plan* P = nullptr;
+            if ((rc = ntt_plan_create(ctx, j.logN, j.moduli, j.limbs, j.domain, nullptr, &P))) return rc;
+            it = b->plans.emplace(std::move(key), std::unique_ptr<ntt_plan, PlanDeleter>(P)).first;
+        }
+        b->job_plan[i] = it->second.get();
+        b->jobs[i].moduli = nullptr;
+        if (fused_supported(j.logN)) by_shape[{j.logN, j.direction == NTT_INVERSE ? 1 : 0}].push_back(i);
+        else two_pass.push_back(i);
+    }
+
+    // descriptor table, one contiguous run per group
+    std::vector<Unit> units;
+    for (auto& kv : by_shape) {
+        FusedGroup g;
+        g.n = kv.first.first;
+        g.inverse = kv.first.second != 0;
+        g.desc_begin = b->desc.size();
+        g.njobs = kv.second.size();
+        size_t total_limbs = 0;
+        uint64_t items = 0;
+        for (size_t i : kv.second) {
+            const ntt_batch_job& j = b->jobs[i];
+            const ntt_plan* P = b->job_plan[i];
+            FusedJob fj{};
+            fj.data = j.data;
+            fj.tw = g.inverse ? P->d_tw_inv : P->d_tw_fwd;
+            fj.lc = g.inverse ? P->d_lc_inv : P->d_lc_fwd;
+            fj.limbs = j.limbs;
+            fj.period = j.limbs;
+            fj.item_begin = (uint32_t)items;
+            fj.n = (uint16_t)g.n;
+            fj.inverse = g.inverse ? 1 : 0;
+            items += fused_items(g.n, j.limbs);
+            if (items > 0xffffffffull) return NTT_ERR_INVALID_ARGUMENT;
+            b->job_group[i] = (int)b->groups.size();
+            b->job_desc[i] = b->desc.size();
+            b->desc.push_back(fj);
+            total_limbs += j.limbs;
+            g.work += 0.5 * (double)((size_t)1 << g.n) * g.n * j.limbs;
+        }
+        g.items = (uint32_t)items;
+        TunedEntry t;
+        if (ctx->tuned_lookup(g.n, total_limbs, &t)) g.eb = t.fused_eb;
+        units.push_back({(int)b->groups.size(), 0, g.work});
+        b->groups.push_back(g);
+    }
+    for (size_t i : two_pass) {
+        const ntt_batch_job& j = b->jobs[i];
+        units.push_back({-1, i, 0.5 * (double)((size_t)1 << j.logN) * j.logN * j.limbs});
+    }
+
+    // longest-processing-time-first over the lanes
+    const size_t nl = std::min<size_t>(units.size(), 1 + BATCH_AUX_STREAMS);
+    b->lanes.assign(nl, {});
+    std::vector<double> load(nl, 0.0);
+    std::stable_sort(units.begin(), units.end(), [](const Unit& a, const Unit& c) { return a.work > c.work; });
+    for (const Unit& u : units) {
+        const size_t l = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->lanes[l].push_back(u);
+        load[l] += u.work;
+    }
+    for (size_t l = 1; l < nl; ++l) {
+        if ((rc = ck(hipStreamCreate(&b->aux[l - 1]), "hipStreamCreate"))) return rc;
+        if ((rc = ck(hipEventCreateWithFlags(&b->join[l - 1], hipEventDisableTiming), "hipEventCreate"))) return rc;
+    }
+    if ((rc = ck(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate"))) return rc;
+
+    if (!b->groups.empty() && (rc = fused_prepare())) return rc;
+    if (!b->desc.empty()) {
+        const size_t bytes = b->desc.size() * sizeof(FusedJob);
+        if (hipMalloc(&b->d_desc, bytes) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
+        if ((rc = ck(hipMemcpy(b->d_desc, b->desc.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(jobs)")))
+            return rc;
+    }
+    *out = b.release();
+    return NTT_OK;
+}
+
+int ntt_batch_destroy(ntt_batch* batch) {
+    if (!batch) return NTT_ERR_INVALID_ARGUMENT;
+    delete batch;
+    return NTT_OK;
+}
+
+int ntt_batch_run(ntt_batch* batch, void* stream) {
+    if (!batch) return NTT_ERR_INVALID_ARGUMENT;
+    return enqueue(batch, (hipStream_t)stream);
+}
+
+int ntt_batch_capture(ntt_batch* batch, void* stream) {
+    if (!batch) return NTT_ERR_INVALID_ARGUMENT;
+    drop_graph(batch);
+    hipStream_t s = (hipStream_t)stream;
+    int rc = ck(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    if (rc) return rc;
+    const int erc = enqueue(batch, s);
+    hipGraph_t g = nullptr;
+    rc = ck(hipStreamEndCapture(s, &g), "hipStreamEndCapture");
+    if (erc || rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return erc ? erc : rc;
+    }
+    batch->graph = g;
+    if ((rc = ck(hipGraphInstantiate(&batch->exec, g, nullptr, nullptr, 0), "hipGraphInstantiate"))) {
+        drop_graph(batch);
+        return rc;
+    }
+    return NTT_OK;
+}
+
+int ntt_batch_replay(ntt_batch* batch, void* stream) {
+    if (!batch) return NTT_ERR_INVALID_ARGUMENT;
+    int rc = NTT_OK;
+    if (!batch->exec && (rc = ntt_batch_capture(batch, stream))) return rc;
+    return ck(hipGraphLaunch(batch->exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+int ntt_batch_set_data(ntt_batch* batch, size_t index, uint32_t* data) {
+    if (!batch || index >= batch->jobs.size() || !data || (reinterpret_cast<uintptr_t>(data) & 15))
+        return NTT_ERR_INVALID_ARGUMENT;
+    batch->jobs[index].data = data;
+    if (batch->job_group[index] < 0) {
+        drop_graph(batch);
+        return NTT_OK;
+    }
+    const size_t d = batch->job_desc[index];
+    batch->desc[d].data = data;
+    // the graph may still be running on another stream: copy after the device is idle
+    int rc = ck(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    if (rc) return rc;
+    return ck(hipMemcpy(batch->d_desc + d, &batch->desc[d], sizeof(FusedJob), hipMemcpyHostToDevice),
+              "hipMemcpy(job)");
+}
+
+int ntt_batch_info(const ntt_batch* batch, int* fused_groups, int* two_pass_jobs, int* streams) {
+    if (!batch) return NTT_ERR_INVALID_ARGUMENT;
+    int tp = 0;
+    for (int g : batch->job_group) tp += g < 0 ? 1 : 0;
+    if (fused_groups) *fused_groups = (int)batch->groups.size();
+    if (two_pass_jobs) *two_pass_jobs = tp;
+    if (streams) *streams = (int)batch->lanes.size();
+    return NTT_OK;
+}
+
+}  // extern "C"

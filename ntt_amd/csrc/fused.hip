@@ -38,6 +38,21 @@ FusedFn fused_fn_eb7(int n, bool inv);   // fused_eb7.hip
 
 static FusedFn fused_fn(int n, bool inv, int eb) { return eb == 6 ? fused_fn_eb6(n, inv) : fused_fn_eb7(n, inv); }
 
+// Raises the dynamic-LDS limit of every fused kernel once per process (before
+// any stream capture that launches them).
+int fused_prepare() {
+    static std::once_flag attr_once;
+    static hipError_t attr_err = hipSuccess;
+    std::call_once(attr_once, [] {
+        const int lds = (int)(FUSED_LDS_WORDS * sizeof(uint32_t));
+        for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
+            for (int d = 0; d < 4 && attr_err == hipSuccess; ++d)
+                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn(m, d & 1, 6 + (d >> 1))),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    });
+    return attr_err == hipSuccess ? NTT_OK : record_hip_error(attr_err, "hipFuncSetAttribute(fused)");
+}
+
 // Launches the persistent kernel of size 2^n / direction `inv`; every job in
 // p has that size and direction.
 int launch_fused(int n, bool inv, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit) {
@@ -45,16 +60,9 @@ int launch_fused(int n, bool inv, int eb, const FusedParams& p, hipStream_t stre
     if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
     FusedFn fn = fused_fn(n, inv, eb);
     if (!fn) return NTT_ERR_CONFIG;
-    static std::once_flag attr_once;
-    static hipError_t attr_err = hipSuccess;
+    int rc = fused_prepare();
+    if (rc) return rc;
     const size_t lds = FUSED_LDS_WORDS * sizeof(uint32_t);
-    std::call_once(attr_once, [&] {
-        for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
-            for (int d = 0; d < 4 && attr_err == hipSuccess; ++d)
-                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn(m, d & 1, 6 + (d >> 1))),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    });
-    if (attr_err != hipSuccess) return record_hip_error(attr_err, "hipFuncSetAttribute(fused)");
     unsigned grid = (unsigned)device_cus();
     if (grid_limit && grid_limit < grid) grid = grid_limit;
     if (p.total_items < grid) grid = p.total_items;

@@ -227,7 +227,7 @@ static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse,
     if (plan_engine(plan, count) == NTT_ENGINE_FUSED) {
         TunedEntry t;
         const bool tuned = !plan->cfg_fixed && plan->ctx->tuned_lookup(plan->logN, plan->limbs * count, &t);
-        const int eb = plan->fused_eb ? plan->fused_eb : (tuned ? t.fused_eb : FUSED_DEFAULT_EB);
+        const int eb = ntt_plan_get_fused_shape(plan, count);
         return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, eb, tuned ? t.stagger : 0);
     }
     return launch_transform(plan_launch(plan, A_dev, count, inverse), (hipStream_t)stream);
@@ -255,6 +255,14 @@ int ntt_plan_set_fused_shape(ntt_plan* plan, int eb) {
     if (eb != 0 && eb != 6 && eb != 7) return NTT_ERR_CONFIG;
     plan->fused_eb = eb;
     return NTT_OK;
+}
+
+int ntt_plan_get_fused_shape(const ntt_plan* plan, size_t count) {
+    if (!plan) return -NTT_ERR_INVALID_ARGUMENT;
+    if (plan->fused_eb) return plan->fused_eb;
+    TunedEntry t;
+    const bool tuned = !plan->cfg_fixed && plan->ctx->tuned_lookup(plan->logN, plan->limbs * count, &t);
+    return tuned ? t.fused_eb : FUSED_DEFAULT_EB;
 }
 
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count) {

@@ -42,6 +42,8 @@ _PROTOS = {
     "ntt_rt_event_synchronize": [_P],
     "ntt_rt_event_elapsed_ms": [ctypes.POINTER(ctypes.c_float), _P, _P],
     "ntt_rt_fill_uniform": [_P, _P, _SZ, _I, _SZ, ctypes.c_uint64, _P],
+    "ntt_rt_butterfly_roof": [_I, ctypes.POINTER(ctypes.c_double), _P],
+    "ntt_rt_stream_copy": [_P, _P, _SZ, _P],
 }
 for _n, _a in _PROTOS.items():
     _f = getattr(_lib, _n)
@@ -185,3 +187,20 @@ def fill_uniform(buf, moduli_dev, limbs: int, logN: int, count: int, seed: int, 
     _ck(_lib.ntt_rt_fill_uniform(buf.ptr if hasattr(buf, "ptr") else buf,
                                  moduli_dev.ptr if hasattr(moduli_dev, "ptr") else moduli_dev,
                                  limbs, logN, count, seed & ((1 << 64) - 1), _sh(stream)), "fill_uniform")
+
+
+BUTTERFLY_CT, BUTTERFLY_CT_LAZY, BUTTERFLY_GS, BUTTERFLY_GS_LAZY = 0, 1, 2, 3
+
+
+def butterfly_roof(variant: int = BUTTERFLY_CT, stream=None) -> float:
+    """Butterflies/s of the kernels' own butterfly code on register-resident
+    operands: the compute roof of the transform kernels."""
+    v = ctypes.c_double(0)
+    _ck(_lib.ntt_rt_butterfly_roof(variant, ctypes.byref(v), _sh(stream)), "butterfly_roof")
+    return v.value
+
+
+def stream_copy(dst, src, nbytes: int, stream=None):
+    """Plain streaming copy (each byte read once, written once)."""
+    _ck(_lib.ntt_rt_stream_copy(dst.ptr if hasattr(dst, "ptr") else dst, src.ptr if hasattr(src, "ptr") else src,
+                                nbytes, _sh(stream)), "stream_copy")
