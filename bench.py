@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the per-transform and single-transform legs")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the batched uneven (256-job) leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default phase config instead of the autotuner's")
     ap.add_argument("--engine", choices=["auto", "fused", "two_pass"], default="auto",
                     help="force a kernel family for the batched leg")
@@ -79,6 +80,89 @@ def hbm_traffic(path: str, kernel: str, workload: str):
         if kv.get("kernel") == kernel and kv.get("workload") == workload:
             return int(kv["fetch_bytes"]) + int(kv["write_bytes"])
     return None
+
+
+def bench_mixed(ntt_amd, dev, stream, reps: int = 10, tune: bool = True, seed: int = 0):
+    """The batched uneven workload: 256 jobs, logN ~ U{12..17}, L ~ U{1..48},
+    direction ~ Bernoulli(1/2), Montgomery domain, seed 0, HBM-resident.
+    Times the captured graph, the eager batch, and one plan launch per job."""
+    import numpy as np
+    from ntt_amd import workloads
+
+    wl = workloads.batched_uneven(seed)
+    ctx = ntt_amd.Context(0)
+    t0 = time.perf_counter()
+    if tune:
+        # tune each size at the launch it will see: one fused group per
+        # (logN, direction) for logN <= 16, single jobs for logN = 17
+        for n in sorted({j.logN for j in wl}):
+            tot = max(sum(j.limbs for j in wl if j.logN == n and j.direction == d) for d in (0, 1))
+            if n <= 16:
+                limbs = min(tot, 48)
+                ctx.autotune(n, workloads.moduli_for(n, limbs), count=max(1, tot // limbs), reps=2)
+            else:
+                ctx.autotune(n, workloads.moduli_for(n, 24), count=1, reps=2)
+    tune_s = time.perf_counter() - t0
+    bufs, spec = [], []
+    qcache = {}
+    for k, j in enumerate(wl):
+        m = workloads.moduli_for(j.logN, j.limbs)
+        buf = dev.DeviceBuffer(4 * j.words)
+        qd = qcache.get((j.logN, j.limbs))
+        if qd is None:
+            qd = qcache[(j.logN, j.limbs)] = dev.DeviceBuffer.from_numpy(np.array([q for q, _ in m], np.uint32))
+        dev.fill_uniform(buf, qd, j.limbs, j.logN, 1, seed=k)
+        bufs.append(buf)
+        spec.append((buf, j.logN, m, j.direction, ntt_amd.MONTGOMERY))
+    dev.synchronize()
+    total_bfly = sum(j.butterflies for j in wl)
+    total_bytes = sum(8 * j.words for j in wl)
+    batch = ctx.batch(spec)
+    batch.capture(stream)
+    e0, e1 = dev.Event(), dev.Event()
+
+    def timed(fn, n):
+        fn()
+        stream.synchronize()
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / n
+
+    graph_ms = timed(lambda: batch.replay(stream), reps)
+    eager_ms = timed(lambda: batch.run(stream), reps)
+    plans = {}
+    for (buf, n, m, d, dom), j in zip(spec, wl):
+        if (n, j.limbs) not in plans:
+            plans[(n, j.limbs)] = ctx.plan(n, m, dom)
+
+    def per_transform():
+        for (buf, n, m, d, dom), j in zip(spec, wl):
+            p = plans[(n, j.limbs)]
+            (p.inverse if d else p.forward)(buf, 1, stream)
+
+    per_ms = timed(per_transform, max(2, reps // 3))
+    info = batch.info()
+    out = {
+        "workload": "256 jobs, logN ~ U{12..17}, limbs ~ U{1..48}, direction ~ Bernoulli(1/2), "
+                    "Montgomery, moduli = first L primes < 2^30 with q = 1 mod 2N, seed 0",
+        "operand_bytes": total_bytes // 2, "butterflies": total_bfly,
+        "graph_ms": round(graph_ms, 4), "graph_gbfly_s": round(total_bfly / (graph_ms * 1e-3) / 1e9, 1),
+        "eager_ms": round(eager_ms, 4), "eager_gbfly_s": round(total_bfly / (eager_ms * 1e-3) / 1e9, 1),
+        "per_transform_ms": round(per_ms, 4),
+        "per_transform_gbfly_s": round(total_bfly / (per_ms * 1e-3) / 1e9, 1),
+        "graph_vs_per_transform": round(per_ms / graph_ms, 3),
+        "hbm_frac": round(total_bytes / (graph_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "fused_groups": info["fused_groups"], "two_pass_jobs": info["two_pass_jobs"],
+        "streams": info["streams"], "twiddle_tables": ctx.twiddle_entries,
+        "twiddle_bytes": ctx.twiddle_bytes, "tune_s": round(tune_s, 2),
+    }
+    batch.close()
+    for b in bufs:
+        b.free()
+    return out
 
 
 def cpu_baseline(logN: int, moduli, seconds: float, threads: int = 1):
@@ -303,6 +387,11 @@ def main():
         })
         log("extras: " + json.dumps(extras))
 
+    mixed = None
+    if not args.no_mixed:
+        mixed = bench_mixed(ntt_amd, dev, stream, tune=not args.no_tune)
+        log("mixed: " + json.dumps(mixed))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(logN, moduli, args.cpu_seconds, threads=1)
@@ -336,6 +425,7 @@ def main():
         "roundtrip_ok": ok,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": mixed,
         "compute_roof": compute_roof,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "autotune": tuning or None,

@@ -281,6 +281,84 @@ int ntt_batch_set_data(ntt_batch* batch, size_t index, uint32_t* data);
  * kernels; streams: concurrent streams used. */
 int ntt_batch_info(const ntt_batch* batch, int* fused_groups, int* two_pass_jobs, int* streams);
 
+/* ---------------------------------------------------------------------------
+ * Serving: a request loop with continuous batching.  No counterpart in the
+ * reference (its main() runs one synchronous transform at a time,
+ * src/v04_montgomery.cu:969-1062).  A scheduler thread owns an operand arena
+ * of hbm_budget_bytes in HBM.  Submitted requests queue in arrival order; a
+ * request is admitted when its operand fits in the arena (first fit,
+ * strictly FIFO, so nothing starves), its operand is placed there (copied
+ * from the given source or generated from a seed), and admitted requests are
+ * launched together on the first free launch slot (one stream each): per
+ * (logN, direction) group one persistent fused kernel, two-pass kernels for
+ * other sizes.  Requests that arrive while launches are in flight are
+ * admitted as arena space and slots free up and run concurrently with them.
+ * Results stay resident in the arena until fetched or released.  Twiddle
+ * tables stay resident in the context's cache.
+ * ------------------------------------------------------------------------- */
+typedef struct ntt_server ntt_server;
+
+typedef struct {
+    size_t hbm_budget_bytes;        /* operand arena size */
+    int slots;                      /* launches in flight at once, 1..8 */
+    int max_jobs_per_launch;        /* 1..4096 */
+} ntt_server_config;
+
+typedef struct {
+    int32_t logN;
+    uint32_t limbs;
+    int32_t direction;              /* ntt_direction */
+    int32_t domain;                 /* ntt_domain */
+    /* host array of `limbs` moduli, or NULL: the `limbs` largest primes
+     * q < 2^30 with q = 1 mod 2N (ntt_generate_moduli) */
+    const ntt_modulus_config* moduli;
+    /* operand source on the host or on the device (one of them, or neither);
+     * it must stay valid until the request is done */
+    const uint32_t* src_host;
+    const uint32_t* src_dev;
+    uint64_t seed;                  /* both sources NULL: uniform residues (ntt_rt_fill_uniform) */
+    int32_t auto_release;           /* 1: free the operand when done (result not kept) */
+} ntt_request;
+
+typedef enum {
+    NTT_REQ_QUEUED = 0, NTT_REQ_ADMITTED = 1, NTT_REQ_RUNNING = 2, NTT_REQ_DONE = 3,
+    NTT_REQ_FAILED = 4, NTT_REQ_UNKNOWN = 5
+} ntt_request_state;
+
+typedef struct {
+    uint64_t submitted, completed, launches, fused_launches, two_pass_launches;
+    double mean_jobs_per_launch;
+    size_t arena_bytes, arena_peak_bytes;
+    int max_slots_busy;             /* most launches seen in flight at once */
+} ntt_server_stats;
+
+int ntt_server_create(ntt_context* ctx, const ntt_server_config* cfg, ntt_server** out);
+/* Stops the scheduler after the work in flight has finished; requests not
+ * yet admitted fail with NTT_ERR_BUSY. */
+int ntt_server_destroy(ntt_server* server);
+/* Queues a request (arrival time = now); NTT_ERR_BUDGET if its operand can
+ * never fit the arena. */
+int ntt_server_submit(ntt_server* server, const ntt_request* req, uint64_t* id);
+int ntt_server_state(ntt_server* server, uint64_t id, int* state);
+/* Blocks until request `id` is done (or failed: its status is returned). */
+int ntt_server_wait(ntt_server* server, uint64_t id);
+/* Copies a finished request's result (limbs x 2^logN words) to `dst` (device
+ * memory if dst_is_device, else host) and, if release, frees its operand. */
+int ntt_server_fetch(ntt_server* server, uint64_t id, uint32_t* dst, int dst_is_device, int release);
+int ntt_server_release(ntt_server* server, uint64_t id);
+int ntt_server_stats_get(ntt_server* server, ntt_server_stats* out);
+/* Builds the plans and twiddle tables for requests of (logN, limbs) with
+ * the default moduli ahead of time, so first requests of a shape do no
+ * host-side setup. */
+int ntt_server_prepare(ntt_server* server, int logN, uint32_t limbs, int domain);
+/* Trace driver: submits request i (default moduli, seeded operand,
+ * auto-release) at arrival_s[i] seconds after the start, waits for all, and
+ * writes each latency (completion seen by the scheduler - arrival) to
+ * latency_s[i]; *wall_s = first arrival to last completion. */
+int ntt_server_run_trace(ntt_server* server, size_t n, const double* arrival_s, const int32_t* logN,
+                         const uint32_t* limbs, const int32_t* direction, const uint64_t* seed,
+                         int domain, double* latency_s, double* wall_s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -131,6 +131,30 @@ class BatchJob(ctypes.Structure):
                 ("logN", ctypes.c_int32), ("direction", ctypes.c_int32), ("domain", ctypes.c_int32)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("slots", ctypes.c_int), ("max_jobs_per_launch", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    """ntt_request of include/ntt_amd.h."""
+    _fields_ = [("logN", ctypes.c_int32), ("limbs", ctypes.c_uint32), ("direction", ctypes.c_int32),
+                ("domain", ctypes.c_int32), ("moduli", ctypes.c_void_p), ("src_host", ctypes.c_void_p),
+                ("src_dev", ctypes.c_void_p), ("seed", ctypes.c_uint64), ("auto_release", ctypes.c_int32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("fused_launches", ctypes.c_uint64), ("two_pass_launches", ctypes.c_uint64),
+                ("mean_jobs_per_launch", ctypes.c_double), ("arena_bytes", ctypes.c_size_t),
+                ("arena_peak_bytes", ctypes.c_size_t), ("max_slots_busy", ctypes.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_QUEUED, REQ_ADMITTED, REQ_RUNNING, REQ_DONE, REQ_FAILED, REQ_UNKNOWN = range(6)
+
+
 @dataclass
 class VersionConfig:
     """VersionConfig{logN, moduli, transform}, src/ntt_config.h:25-29."""
@@ -197,6 +221,16 @@ _PROTOS = {
     "ntt_batch_replay": (_I, [_P, _P]),
     "ntt_batch_set_data": (_I, [_P, _SZ, _P]),
     "ntt_batch_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ntt_server_create": (_I, [_P, _P, ctypes.POINTER(_P)]),
+    "ntt_server_destroy": (_I, [_P]),
+    "ntt_server_submit": (_I, [_P, _P, ctypes.POINTER(ctypes.c_uint64)]),
+    "ntt_server_state": (_I, [_P, ctypes.c_uint64, ctypes.POINTER(_I)]),
+    "ntt_server_wait": (_I, [_P, ctypes.c_uint64]),
+    "ntt_server_fetch": (_I, [_P, ctypes.c_uint64, _P, _I, _I]),
+    "ntt_server_release": (_I, [_P, ctypes.c_uint64]),
+    "ntt_server_stats_get": (_I, [_P, _P]),
+    "ntt_server_prepare": (_I, [_P, _I, ctypes.c_uint32, _I]),
+    "ntt_server_run_trace": (_I, [_P, _SZ, _P, _P, _P, _P, _P, _I, _P, ctypes.POINTER(ctypes.c_double)]),
 }
 
 
@@ -436,6 +470,97 @@ class Context:
 
     def batch(self, jobs) -> "Batch":
         return Batch(self, jobs)
+
+    def server(self, hbm_budget_bytes: int = 64 << 30, slots: int = 4, max_jobs_per_launch: int = 256) -> "Server":
+        return Server(self, hbm_budget_bytes, slots, max_jobs_per_launch)
+
+
+class Server:
+    """Request loop with continuous batching (ntt_server_*)."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int, slots: int, max_jobs_per_launch: int):
+        self.ctx = ctx
+        cfg = ServerConfig(hbm_budget_bytes, slots, max_jobs_per_launch)
+        h = ctypes.c_void_p()
+        _check(_lib.ntt_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(h)), "ntt_server_create")
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, logN: int, limbs: int, direction: int = FORWARD, domain: int = MONTGOMERY, moduli=None,
+               src_host: Optional[np.ndarray] = None, src_dev=None, seed: int = 0, auto_release: bool = False) -> int:
+        r = Request(logN, limbs, direction, domain, None, None, None, seed & ((1 << 64) - 1), int(auto_release))
+        keep = []
+        if moduli is not None:
+            ms = as_moduli(moduli)
+            arr = (ModulusConfig * len(ms))(*ms)
+            keep.append(arr)
+            r.moduli = ctypes.addressof(arr)
+        if src_host is not None:
+            a = np.ascontiguousarray(src_host, dtype=np.uint32)
+            keep.append(a)
+            r.src_host = a.ctypes.data
+        if src_dev is not None:
+            r.src_dev = _addr(src_dev)
+        rid = ctypes.c_uint64(0)
+        _check(_lib.ntt_server_submit(self.handle, ctypes.byref(r), ctypes.byref(rid)), "ntt_server_submit")
+        if keep:
+            self._keep[rid.value] = keep      # sources must outlive the request
+        return rid.value
+
+    def state(self, rid: int) -> int:
+        v = _I(0)
+        _check(_lib.ntt_server_state(self.handle, rid, ctypes.byref(v)), "ntt_server_state")
+        return v.value
+
+    def wait(self, rid: int):
+        _check(_lib.ntt_server_wait(self.handle, rid), "ntt_server_wait")
+
+    def fetch(self, rid: int, logN: int, limbs: int, release: bool = True) -> np.ndarray:
+        out = np.empty((limbs, 1 << logN), np.uint32)
+        _check(_lib.ntt_server_fetch(self.handle, rid, out.ctypes.data, 0, int(release)), "ntt_server_fetch")
+        if release:
+            self._keep.pop(rid, None)
+        return out
+
+    def release(self, rid: int):
+        _check(_lib.ntt_server_release(self.handle, rid), "ntt_server_release")
+        self._keep.pop(rid, None)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _check(_lib.ntt_server_stats_get(self.handle, ctypes.byref(st)), "ntt_server_stats_get")
+        return st.as_dict()
+
+    def prepare(self, logN: int, limbs: int, domain: int = MONTGOMERY):
+        _check(_lib.ntt_server_prepare(self.handle, logN, limbs, domain), "ntt_server_prepare")
+
+    def run_trace(self, requests, domain: int = MONTGOMERY):
+        """Replays workloads.Request records (arrival times in seconds from the
+        start); returns (latency seconds per request, wall seconds)."""
+        n = len(requests)
+        arr = np.array([r.arrival_s for r in requests], np.float64)
+        ln = np.array([r.job.logN for r in requests], np.int32)
+        lb = np.array([r.job.limbs for r in requests], np.uint32)
+        dr = np.array([r.job.direction for r in requests], np.int32)
+        sd = np.array([r.seed for r in requests], np.uint64)
+        lat = np.zeros(n, np.float64)
+        wall = ctypes.c_double(0)
+        _check(_lib.ntt_server_run_trace(self.handle, n, arr.ctypes.data, ln.ctypes.data, lb.ctypes.data,
+                                         dr.ctypes.data, sd.ctypes.data, domain, lat.ctypes.data,
+                                         ctypes.byref(wall)), "ntt_server_run_trace")
+        return lat, wall.value
+
+    def close(self):
+        if self.handle:
+            _lib.ntt_server_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:
