@@ -52,6 +52,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the per-transform and single-transform legs")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-stream leg")
+    ap.add_argument("--served-requests", type=int, default=20000)
     ap.add_argument("--no-mixed", action="store_true", help="skip the batched uneven (256-job) leg")
     ap.add_argument("--no-tune", action="store_true", help="use the default phase config instead of the autotuner's")
     ap.add_argument("--engine", choices=["auto", "fused", "two_pass"], default="auto",
@@ -165,6 +167,54 @@ def bench_mixed(ntt_amd, dev, stream, reps: int = 10, tune: bool = True, seed: i
     return out
 
 
+def bench_served(ntt_amd, requests: int = 20000, fractions=(0.25, 0.5, 0.75, 0.9, 1.0, 1.1),
+                 headline: float = 0.75, seed: int = 0, budget_gb: float = 64.0):
+    """The served stream: `requests` requests, logN ~ U{13..17}, L log-uniform
+    on [4, 64], direction ~ Bernoulli(1/2), Montgomery, seeded operands
+    generated on the device at admission, 64 GB operand budget.  First the
+    saturation throughput (every request arriving at t = 0), then Poisson
+    arrivals at the given fractions of it."""
+    import numpy as np
+    from ntt_amd import workloads
+
+    ctx = ntt_amd.Context(0)
+    srv = ctx.server(hbm_budget_bytes=int(budget_gb * (1 << 30)), slots=4, max_jobs_per_launch=256)
+    base = workloads.serving_stream(seed, requests, rate_per_s=1.0)
+    burst = [workloads.Request(0.0, r.job, r.seed) for r in base]
+    bfly = sum(r.job.butterflies for r in base)
+    srv.run_trace(burst[:2000])                      # warm-up: plans, tables, kernels
+    lat, wall = srv.run_trace(burst)
+    sat_rps = requests / wall
+    out = {"workload": f"{requests} requests, logN ~ U{{13..17}}, limbs log-uniform on [4, 64] (rounded), "
+                       "direction ~ Bernoulli(1/2), Montgomery, Poisson arrivals, operands generated on the "
+                       "device at admission, 64 GB operand budget, 4 launch slots",
+           "saturation_req_s": round(sat_rps, 1), "saturation_gbfly_s": round(bfly / wall / 1e9, 1),
+           "saturation_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "saturation_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3), "sweep": []}
+    for f in fractions:
+        rate = f * sat_rps
+        reqs = workloads.serving_stream(seed + 1, requests, rate_per_s=rate)
+        st0 = srv.stats()
+        lat, wall = srv.run_trace(reqs)
+        st1 = srv.stats()
+        launches = st1["launches"] - st0["launches"]
+        row = {"fraction": f, "rate_req_s": round(rate, 1), "throughput_req_s": round(requests / wall, 1),
+               "gbfly_s": round(sum(r.job.butterflies for r in reqs) / wall / 1e9, 1),
+               "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+               "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+               "jobs_per_launch": round(requests / max(1, launches), 2)}
+        out["sweep"].append(row)
+        log(f"served {row}")
+        if f == headline:
+            out.update({"p50_ms": row["p50_ms"], "p99_ms": row["p99_ms"], "at_fraction": f,
+                        "throughput_req_s": row["throughput_req_s"], "gbfly_s": row["gbfly_s"]})
+    st = srv.stats()
+    out["arena_peak_bytes"] = st["arena_peak_bytes"]
+    out["max_slots_busy"] = st["max_slots_busy"]
+    srv.close()
+    return out
+
+
 def cpu_baseline(logN: int, moduli, seconds: float, threads: int = 1):
     """Oracle butterfly loops on host cores over a bounded sample: forward +
     inverse of whole limbs of the workload (seed 0 inputs), until `seconds`."""
@@ -199,9 +249,14 @@ def cpu_baseline(logN: int, moduli, seconds: float, threads: int = 1):
             limbs_done += threads
     dt = time.perf_counter() - t0
     bfly = 2 * 0.5 * (1 << logN) * logN * limbs_done
+    model = ""
+    try:
+        model = next(ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
     return {"value": bfly / dt / 1e9, "unit": "Gbutterfly/s", "cores": threads, "kind": "port",
             "sample": f"{limbs_done} limbs of N=2^{logN} (forward + inverse each, seed-0 inputs, "
-                      f"tables prebuilt), {dt:.1f} s"}
+                      f"tables prebuilt), {dt:.1f} s, {threads} thread(s) on {model}"}
 
 
 def main():
@@ -392,10 +447,20 @@ def main():
         mixed = bench_mixed(ntt_amd, dev, stream, tune=not args.no_tune)
         log("mixed: " + json.dumps(mixed))
 
-    cpu = None
+    served = None
+    if not args.no_served:
+        served = bench_served(ntt_amd, requests=args.served_requests)
+        log("served: " + json.dumps({k: v for k, v in served.items() if k != "sweep"}))
+
+    cpu = cpu1 = None
     if not args.no_cpu:
-        cpu = cpu_baseline(logN, moduli, args.cpu_seconds, threads=1)
-        log(f"cpu baseline: {cpu}")
+        # limbs are independent: spread over the host threads this job may use
+        # (at most 16, the CPU share of one GPU on the box)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        cpu = cpu_baseline(logN, moduli, args.cpu_seconds, threads=threads)
+        log(f"cpu baseline ({threads} threads): {cpu}")
+        cpu1 = cpu_baseline(logN, moduli, args.cpu_seconds / 2, threads=1)
+        log(f"cpu baseline (1 thread): {cpu1}")
 
     out = {
         "metric": METRIC,
@@ -422,10 +487,14 @@ def main():
             "parallelism": "single GPU",
         },
         "hbm_frac_transform": round(hbm_frac_transform, 4),
+        "served_p50_ms": served["p50_ms"] if served else None,
+        "served_p99_ms": served["p99_ms"] if served else None,
         "roundtrip_ok": ok,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
         "mixed": mixed,
+        "served": served,
         "compute_roof": compute_roof,
         "kernel_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "autotune": tuning or None,
