@@ -46,6 +46,23 @@ def _compile(src: str) -> str:
     return obj
 
 
+ROOT = os.path.dirname(HERE)
+COMPAT_SRC = os.path.join(ROOT, "tests", "compat", "drop_in_main.cpp")
+COMPAT_BIN = os.path.join(ROOT, "tests", "compat", "_bin", "drop_in_main")
+
+
+def build_compat_example() -> str:
+    """The reference-style caller of include/ntt_amd_compat.hpp (host C++ only,
+    g++), linked against the library; used by tests/test_compat.py."""
+    os.makedirs(os.path.dirname(COMPAT_BIN), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), COMPAT_SRC,
+           "-L", HERE, "-lntt_amd", "-Wl,-rpath,$ORIGIN/../../../ntt_amd", "-o", COMPAT_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compat example build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return COMPAT_BIN
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -57,6 +74,7 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_compat_example()
     if verbose:
         print(f"built {LIB}")
     return LIB
