@@ -343,11 +343,12 @@ def main():
         log("ROUNDTRIP MISMATCH after the timed region")
 
     # ---- kernel durations of the engine in use (HIP events on the launch stream) ----
-    eb = plan.fused_shape(B)
+    ebf, lzf = plan.fused_variant(B, ntt_amd.FORWARD)
+    ebi, _ = plan.fused_variant(B, ntt_amd.INVERSE)
     lazy = all(q < (1 << 30) for q, _ in moduli)   # two-pass kernels then use lazy reduction
     if engine_used == "fused":
-        kern = {f"fused_kernel<{logN}, false, false, {eb}>": lambda: plan.forward(pool, B, stream),
-                f"fused_kernel<{logN}, true, false, {eb}>": lambda: plan.inverse(pool, B, stream)}
+        kern = {f"fused_kernel<{logN}, false, {'true' if lzf else 'false'}, {ebf}>": lambda: plan.forward(pool, B, stream),
+                f"fused_kernel<{logN}, true, false, {ebi}>": lambda: plan.inverse(pool, B, stream)}
     else:
         mode = 2 if lazy else 0
         c = cfg.astuple()
@@ -379,9 +380,9 @@ def main():
     log("kernel ms: " + ", ".join(f"{n} {v:.4f}" for n, v in kernel_ms.items()))
 
     # the binding compute roof: the same butterfly code on register-resident operands
-    inverse_dom = ", true," in dom
-    variant = (dev.BUTTERFLY_GS if inverse_dom else dev.BUTTERFLY_CT) + (1 if (lazy and engine_used != "fused")
-                                                                          else 0)
+    inverse_dom = dom.split("<", 1)[1].split(",")[1].strip() == "true"   # second template argument: INV
+    lazy_dom = (lazy and engine_used != "fused") or (engine_used == "fused" and not inverse_dom and lzf)
+    variant = (dev.BUTTERFLY_GS if inverse_dom else dev.BUTTERFLY_CT) + (1 if lazy_dom else 0)
     roof_bfly = dev.butterfly_roof(variant, stream)
     bfly_dom = B * bfly_per_transform / (kernel_ms[dom] * 1e-3)
     compute_roof = {"bound": "valu", "unit": "Gbutterfly/s", "achieved": round(bfly_dom / 1e9, 1),
@@ -481,7 +482,7 @@ def main():
                         f"({4 * words_per_op * B / 2**30:.2f} GiB, HBM-resident), batched launch",
             "logN": logN, "limbs": limbs, "batch": B,
             "engine": engine_used,
-            "fused_shape": eb if engine_used == "fused" else None,
+            "fused_variant": {"forward": [ebf, lzf], "inverse": [ebi, False]} if engine_used == "fused" else None,
             "phase_config": cfg.astuple(),
             "chunk_operands": tuning["batch"]["chunk"] if tuning else 0,
             "parallelism": "single GPU",

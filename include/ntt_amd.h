@@ -190,11 +190,17 @@ int ntt_plan_destroy(ntt_plan* plan);
 int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out);
 /* Selects the kernel family (ntt_engine); NTT_ERR_CONFIG if unsupported for logN. */
 int ntt_plan_set_engine(ntt_plan* plan, int engine);
-/* Workgroup shape of the fused engine: 7 = 512 threads x 128 words each,
- * 6 = 1024 threads x 64 words, 0 = the tuned choice (default 7). */
+/* Workgroup shape of the fused engine for both directions: 7 = 512 threads x
+ * 128 words each, 6 = 1024 threads x 64 words, 0 = the tuned choice
+ * (default: 7 forward, 6 inverse). */
 int ntt_plan_set_fused_shape(ntt_plan* plan, int eb);
-/* The fused shape a run of `count` operands would use (6 or 7). */
-int ntt_plan_get_fused_shape(const ntt_plan* plan, size_t count);
+/* Arithmetic of the fused forward kernel: 1 = lazy reduction (used only when
+ * every q < 2^30), 0 = canonical, -1 = the tuned choice (default lazy).  The
+ * inverse fused kernel is always canonical.  Outputs are canonical either way. */
+int ntt_plan_set_fused_arith(ntt_plan* plan, int lazy);
+/* The fused kernel variant (shape, lazy) a run of `count` operands uses in
+ * `direction`. */
+int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction, int* eb, int* lazy);
 /* Engine a launch of `count` operands would use (AUTO resolved). */
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count);
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream);
@@ -229,7 +235,9 @@ typedef struct {
     int engine;                 /* ntt_engine chosen (TWO_PASS or FUSED) */
     double fused_fwd_ms, fused_inv_ms;       /* best fused engine times (0 if not applicable) */
     int fused_stagger;          /* start stagger chosen for the fused engine */
-    int fused_eb;               /* fused engine shape: 7 = 512 threads x 128 words, 6 = 1024 x 64 */
+    int fused_eb;               /* fused forward shape: 7 = 512 threads x 128 words, 6 = 1024 x 64 */
+    int fused_eb_inv;           /* fused inverse shape */
+    int fused_lazy;             /* fused forward arithmetic: 1 lazy, 0 canonical */
 } ntt_tune_result;
 
 int ntt_autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs,

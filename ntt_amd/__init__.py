@@ -115,14 +115,15 @@ class TuneResult(ctypes.Structure):
                 ("inv_ms", ctypes.c_double), ("default_fwd_ms", ctypes.c_double),
                 ("default_inv_ms", ctypes.c_double), ("candidates", ctypes.c_int), ("engine", ctypes.c_int),
                 ("fused_fwd_ms", ctypes.c_double), ("fused_inv_ms", ctypes.c_double), ("fused_stagger", ctypes.c_int),
-                ("fused_eb", ctypes.c_int)]
+                ("fused_eb", ctypes.c_int), ("fused_eb_inv", ctypes.c_int), ("fused_lazy", ctypes.c_int)]
 
     def as_dict(self):
         return {"cfg": self.cfg.astuple(), "chunk": int(self.chunk), "fwd_ms": self.fwd_ms, "inv_ms": self.inv_ms,
                 "default_fwd_ms": self.default_fwd_ms, "default_inv_ms": self.default_inv_ms,
                 "candidates": self.candidates, "engine": {1: "two_pass", 2: "fused"}.get(self.engine, self.engine),
                 "fused_fwd_ms": self.fused_fwd_ms, "fused_inv_ms": self.fused_inv_ms,
-                "fused_stagger": self.fused_stagger, "fused_eb": self.fused_eb}
+                "fused_stagger": self.fused_stagger, "fused_eb": self.fused_eb, "fused_eb_inv": self.fused_eb_inv,
+                "fused_lazy": self.fused_lazy}
 
 
 class BatchJob(ctypes.Structure):
@@ -207,7 +208,8 @@ _PROTOS = {
     "ntt_plan_run_pass": (_I, [_P, _P, _SZ, _I, _I, _P]),
     "ntt_plan_set_engine": (_I, [_P, _I]),
     "ntt_plan_set_fused_shape": (_I, [_P, _I]),
-    "ntt_plan_get_fused_shape": (_I, [_P, _SZ]),
+    "ntt_plan_set_fused_arith": (_I, [_P, _I]),
+    "ntt_plan_get_fused_variant": (_I, [_P, _SZ, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ntt_plan_get_engine": (_I, [_P, _SZ]),
     "ntt_autotune": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _SZ, _I, _I, ctypes.POINTER(TuneResult)]),
     "ntt_context_set_tuning": (_I, [_P, _I, _SZ, _PCFG, _SZ, _I]),
@@ -636,12 +638,19 @@ class Plan:
     def set_engine(self, engine: int):
         _check(_lib.ntt_plan_set_engine(self.handle, engine), "ntt_plan_set_engine")
 
-    def fused_shape(self, count: int = 1) -> int:
-        """Fused workgroup shape a run of `count` operands uses."""
-        return _lib.ntt_plan_get_fused_shape(self.handle, count)
+    def fused_variant(self, count: int = 1, direction: int = FORWARD):
+        """(shape, lazy) of the fused kernel a run of `count` operands uses."""
+        eb, lz = _I(0), _I(0)
+        _check(_lib.ntt_plan_get_fused_variant(self.handle, count, direction, ctypes.byref(eb), ctypes.byref(lz)),
+               "ntt_plan_get_fused_variant")
+        return eb.value, bool(lz.value)
+
+    def set_fused_arith(self, lazy: int):
+        """Fused forward arithmetic: 1 lazy (when every q < 2^30), 0 canonical, -1 tuned."""
+        _check(_lib.ntt_plan_set_fused_arith(self.handle, lazy), "ntt_plan_set_fused_arith")
 
     def set_fused_shape(self, eb: int):
-        """Fused workgroup shape: 7 (512 threads x 128 words), 6 (1024 x 64), 0 = tuned."""
+        """Fused workgroup shape, both directions: 7 (512 threads x 128 words), 6 (1024 x 64), 0 = tuned."""
         _check(_lib.ntt_plan_set_fused_shape(self.handle, eb), "ntt_plan_set_fused_shape")
 
     def engine(self, count: int = 1) -> int:

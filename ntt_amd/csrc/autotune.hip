@@ -158,7 +158,8 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
     size_t best_chunk = 0;
     double def_fwd = 0, def_inv = 0, fused_fwd = 0, fused_inv = 0;
     int engine = NTT_ENGINE_TWO_PASS;
-    int fused_eb = FUSED_DEFAULT_EB;
+    int fused_eb_fwd = FUSED_DEFAULT_EB_FWD, fused_eb_inv = FUSED_DEFAULT_EB_INV;
+    bool fused_lazy = P.lazy;
     int stagger = 0;
     if (rc == NTT_OK) {
         // chunking of multi-operand launches (whole transforms, both passes)
@@ -182,31 +183,40 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
                 best_inv = i;
             }
         }
-        // the fused limb kernel where it applies: both workgroup shapes, and the
-        // start stagger (only worth it with several items per CU)
-        const bool many = fused_supported(logN) &&
-                          (limbs * count) >> (16 - logN) >= 2 * (size_t)device_cus();
-        for (int eb = 7; rc == NTT_OK && fused_supported(logN) && eb >= 6; --eb) {
-            for (int st : {0, 2, 4, 6, 8}) {
-                if (st && !many) break;
-                double f = 0, i = 0;
-                rc = time_median(s, reps, [&] { return plan_launch_fused(&P, buf, count, false, s, eb, st); }, &f);
+        // the fused limb kernel where it applies: each direction on its own,
+        // over both workgroup shapes, and for the forward direction both
+        // arithmetics when every q < 2^30
+        if (rc == NTT_OK && fused_supported(logN)) {
+            double bf = 0, bi = 0;
+            for (int eb = 7; rc == NTT_OK && eb >= 6; --eb) {
+                for (int lz = P.lazy ? 1 : 0; rc == NTT_OK && lz >= 0; --lz) {
+                    double f = 0;
+                    rc = time_median(s, reps, [&] { return plan_launch_fused(&P, buf, count, false, s, eb, lz != 0, 0); },
+                                     &f);
+                    ++ncand;
+                    if (rc == NTT_OK && (bf == 0 || f < bf)) {
+                        bf = f;
+                        fused_eb_fwd = eb;
+                        fused_lazy = lz != 0;
+                    }
+                }
+                double i = 0;
                 if (rc == NTT_OK)
-                    rc = time_median(s, reps, [&] { return plan_launch_fused(&P, buf, count, true, s, eb, st); }, &i);
-                if (rc) break;
+                    rc = time_median(s, reps, [&] { return plan_launch_fused(&P, buf, count, true, s, eb, false, 0); },
+                                     &i);
                 ++ncand;
-                if (fused_fwd == 0 || f + i < fused_fwd + fused_inv) {
-                    fused_fwd = f;
-                    fused_inv = i;
+                if (rc == NTT_OK && (bi == 0 || i < bi)) {
+                    bi = i;
+                    fused_eb_inv = eb;
                 }
-                if (f + i < best_fwd + best_inv) {
-                    engine = NTT_ENGINE_FUSED;
-                    fused_eb = eb;
-                    stagger = st;
-                    best_fwd = f;
-                    best_inv = i;
-                    best_chunk = 0;
-                }
+            }
+            fused_fwd = bf;
+            fused_inv = bi;
+            if (rc == NTT_OK && bf + bi < best_fwd + best_inv) {
+                engine = NTT_ENGINE_FUSED;
+                best_fwd = bf;
+                best_inv = bi;
+                best_chunk = 0;
             }
         }
         // the default config, for comparison
@@ -229,7 +239,9 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
     TunedEntry e;
     e.cfg = best;
     e.engine = engine;
-    e.fused_eb = fused_eb;
+    e.fused_eb_fwd = fused_eb_fwd;
+    e.fused_eb_inv = fused_eb_inv;
+    e.fused_lazy = fused_lazy;
     e.stagger = stagger;
     e.chunk = best_chunk;
     e.fwd_ms = best_fwd;
@@ -250,7 +262,9 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
         out->fused_fwd_ms = fused_fwd;
         out->fused_inv_ms = fused_inv;
         out->fused_stagger = stagger;
-        out->fused_eb = fused_eb;
+        out->fused_eb = fused_eb_fwd;
+        out->fused_eb_inv = fused_eb_inv;
+        out->fused_lazy = fused_lazy ? 1 : 0;
     }
     return NTT_OK;
 }
@@ -297,13 +311,15 @@ int ntt_context_save_tuning(const ntt_context* ctx, const char* path) {
     FILE* f = std::fopen(path, "w");
     if (!f) return NTT_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> g(ctx->tune_mu);
-    std::fprintf(f, "# ntt_amd tuning v1: logN limbs_log2 chunk r s w x4 fwd_ms inv_ms engine fused_eb stagger\n");
+    std::fprintf(f, "# ntt_amd tuning v2: logN limbs_log2 chunk r s w x4 fwd_ms inv_ms engine fused_eb_fwd "
+                    "fused_eb_inv fused_lazy stagger\n");
     for (auto& kv : ctx->tuned) {
         const TunedEntry& e = kv.second;
         const ntt_phase_config* p = &e.cfg.ntt_phase1;
         std::fprintf(f, "%d %d %zu", kv.first.first, kv.first.second, e.chunk);
         for (int i = 0; i < 4; ++i) std::fprintf(f, " %d %d %d", p[i].radix_stages, p[i].stage_merging, p[i].warp_batching);
-        std::fprintf(f, " %.6f %.6f %d %d %d\n", e.fwd_ms, e.inv_ms, e.engine, e.fused_eb, e.stagger);
+        std::fprintf(f, " %.6f %.6f %d %d %d %d %d\n", e.fwd_ms, e.inv_ms, e.engine, e.fused_eb_fwd, e.fused_eb_inv,
+                     e.fused_lazy ? 1 : 0, e.stagger);
     }
     std::fclose(f);
     return NTT_OK;
@@ -320,10 +336,10 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         int logN, lb, v[12];
         size_t chunk;
         double fm = 0, im = 0;
-        int eng = NTT_ENGINE_TWO_PASS, feb = FUSED_DEFAULT_EB, stag = 0;
-        int n = std::sscanf(line, "%d %d %zu %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d %d %d", &logN, &lb,
+        int eng = NTT_ENGINE_TWO_PASS, febf = FUSED_DEFAULT_EB_FWD, febi = FUSED_DEFAULT_EB_INV, lz = 1, stag = 0;
+        int n = std::sscanf(line, "%d %d %zu %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d %d %d %d %d", &logN, &lb,
                             &chunk, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10],
-                            &v[11], &fm, &im, &eng, &feb, &stag);
+                            &v[11], &fm, &im, &eng, &febf, &febi, &lz, &stag);
         if (n < 15) { rc = NTT_ERR_INVALID_ARGUMENT; break; }
         TunedEntry e;
         ntt_phase_config* p = &e.cfg.ntt_phase1;
@@ -331,7 +347,9 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         if (validate_config(logN, &e.cfg, NTT_LDS_BYTES_GFX950) != NTT_OK) { rc = NTT_ERR_CONFIG; break; }
         e.chunk = chunk;
         e.engine = (eng == NTT_ENGINE_FUSED && fused_supported(logN)) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
-        e.fused_eb = feb == 6 ? 6 : 7;
+        e.fused_eb_fwd = febf == 6 ? 6 : 7;
+        e.fused_eb_inv = febi == 6 ? 6 : 7;
+        e.fused_lazy = lz != 0;
         e.stagger = stag < 0 ? 0 : (stag > 64 ? 64 : stag);
         e.fwd_ms = fm;
         e.inv_ms = im;

@@ -25,7 +25,7 @@
 #include "launch.hpp"
 
 namespace ntt {
-int launch_fused(int n, bool inv, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
+int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
 int fused_prepare();
 uint32_t fused_items(int logN, size_t limbs);
 }  // namespace ntt
@@ -43,7 +43,8 @@ struct PlanDeleter {
 struct FusedGroup {
     int n = 0;
     bool inverse = false;
-    int eb = FUSED_DEFAULT_EB;
+    int eb = FUSED_DEFAULT_EB_FWD;
+    bool lazy = false;              // lazy forward kernel (every job's q < 2^30)
     size_t desc_begin = 0;          // first descriptor in the batch's table
     size_t njobs = 0;
     uint32_t items = 0;
@@ -110,7 +111,7 @@ int enqueue(ntt_batch* b, hipStream_t stream) {
                 p.total_items = g.items;
                 p.counter = nullptr;          // static round-robin: nothing to reset between replays
                 p.stagger = 0;
-                rc = launch_fused(g.n, g.inverse, g.eb, p, s, 0);
+                rc = launch_fused(g.n, g.inverse, g.eb, g.lazy, p, s, 0);
             } else {
                 const ntt_batch_job& j = b->jobs[u.job];
                 rc = launch_transform(plan_launch(b->job_plan[u.job], j.data, 1, j.direction == NTT_INVERSE), s);
@@ -150,7 +151,7 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
     b->job_group.assign(njobs, -1);
     b->job_desc.assign(njobs, 0);
     int rc = NTT_OK;
-    std::map<std::pair<int, int>, std::vector<size_t>> by_shape;   // (logN, inverse) -> fused jobs
+    std::map<std::tuple<int, int, int>, std::vector<size_t>> by_shape;   // (logN, inverse, lazy-capable) -> jobs
     std::vector<size_t> two_pass;
     for (size_t i = 0; i < njobs; ++i) {
         const ntt_batch_job& j = jobs[i];
@@ -172,7 +173,8 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         }
         b->job_plan[i] = it->second.get();
         b->jobs[i].moduli = nullptr;
-        if (fused_supported(j.logN)) by_shape[{j.logN, j.direction == NTT_INVERSE ? 1 : 0}].push_back(i);
+        if (fused_supported(j.logN))
+            by_shape[std::make_tuple(j.logN, j.direction == NTT_INVERSE ? 1 : 0, it->second->lazy ? 1 : 0)].push_back(i);
         else two_pass.push_back(i);
     }
 
@@ -180,8 +182,9 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
     std::vector<Unit> units;
     for (auto& kv : by_shape) {
         FusedGroup g;
-        g.n = kv.first.first;
-        g.inverse = kv.first.second != 0;
+        g.n = std::get<0>(kv.first);
+        g.inverse = std::get<1>(kv.first) != 0;
+        const bool lazy_capable = std::get<2>(kv.first) != 0;
         g.desc_begin = b->desc.size();
         g.njobs = kv.second.size();
         size_t total_limbs = 0;
@@ -207,8 +210,7 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
             g.work += 0.5 * (double)((size_t)1 << g.n) * g.n * j.limbs;
         }
         g.items = (uint32_t)items;
-        TunedEntry t;
-        if (ctx->tuned_lookup(g.n, total_limbs, &t)) g.eb = t.fused_eb;
+        fused_variant_for(ctx, g.n, total_limbs, g.inverse, lazy_capable, &g.eb, &g.lazy);
         units.push_back({(int)b->groups.size(), 0, g.work});
         b->groups.push_back(g);
     }

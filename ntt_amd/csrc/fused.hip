@@ -33,10 +33,12 @@ uint32_t fused_items(int logN, size_t limbs) {
 }
 
 
-FusedFn fused_fn_eb6(int n, bool inv);   // fused_eb6.hip
-FusedFn fused_fn_eb7(int n, bool inv);   // fused_eb7.hip
+FusedFn fused_fn_eb6(int n, bool inv, bool lazy);   // fused_eb6.hip
+FusedFn fused_fn_eb7(int n, bool inv, bool lazy);   // fused_eb7.hip
 
-static FusedFn fused_fn(int n, bool inv, int eb) { return eb == 6 ? fused_fn_eb6(n, inv) : fused_fn_eb7(n, inv); }
+static FusedFn fused_fn(int n, bool inv, int eb, bool lazy) {
+    return eb == 6 ? fused_fn_eb6(n, inv, lazy) : fused_fn_eb7(n, inv, lazy);
+}
 
 // Raises the dynamic-LDS limit of every fused kernel once per process (before
 // any stream capture that launches them).
@@ -46,19 +48,23 @@ int fused_prepare() {
     std::call_once(attr_once, [] {
         const int lds = (int)(FUSED_LDS_WORDS * sizeof(uint32_t));
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
-            for (int d = 0; d < 4 && attr_err == hipSuccess; ++d)
-                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn(m, d & 1, 6 + (d >> 1))),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            for (int d = 0; d < 8 && attr_err == hipSuccess; ++d) {
+                FusedFn f = fused_fn(m, d & 1, 6 + ((d >> 1) & 1), d >= 4);
+                if (f)
+                    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            }
     });
     return attr_err == hipSuccess ? NTT_OK : record_hip_error(attr_err, "hipFuncSetAttribute(fused)");
 }
 
 // Launches the persistent kernel of size 2^n / direction `inv`; every job in
 // p has that size and direction.
-int launch_fused(int n, bool inv, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit) {
+int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream,
+                 unsigned grid_limit) {
     if (p.total_items == 0) return NTT_OK;
     if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
-    FusedFn fn = fused_fn(n, inv, eb);
+    FusedFn fn = fused_fn(n, inv, eb, lazy);
     if (!fn) return NTT_ERR_CONFIG;
     int rc = fused_prepare();
     if (rc) return rc;
@@ -75,7 +81,7 @@ int launch_fused(int n, bool inv, int eb, const FusedParams& p, hipStream_t stre
 
 // `count` operands of a plan as one job (limb l uses table l % limbs).
 int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream, int eb,
-                      int stagger) {
+                      bool lazy, int stagger) {
     if (!fused_supported(P->logN)) return NTT_ERR_CONFIG;
     if (reinterpret_cast<uintptr_t>(A) & 15) return NTT_ERR_INVALID_ARGUMENT;
     const size_t limbs = P->limbs * count;
@@ -94,7 +100,8 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.total_items = fused_items(P->logN, limbs);
     p.counter = nullptr;
     p.stagger = stagger < 0 ? 0u : (uint32_t)stagger;
-    return launch_fused(P->logN, inverse, eb, p, stream, 0);
+    // lazy reduction needs every q < 2^30 and exists for the forward kernels
+    return launch_fused(P->logN, inverse, eb, lazy && P->lazy && !inverse, p, stream, 0);
 }
 
 }  // namespace ntt

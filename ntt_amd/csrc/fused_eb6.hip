@@ -1,14 +1,28 @@
 // Instantiations of the fused limb kernel with window width 6
-// (1024 threads x 64 words per thread); one translation unit per
-// shape, so the two compile in parallel.
+// (1024 threads x 64 words per thread): canonical arithmetic in
+// both directions, lazy reduction (q < 2^30) for the forward transform.  One
+// translation unit per shape, so the two compile in parallel.
 #include <hip/hip_runtime.h>
 
 #include "fused_kernel.hpp"
 
 namespace ntt {
 
-FusedFn fused_fn_eb6(int n, bool inv) {
+FusedFn fused_fn_eb6(int n, bool inv, bool lazy) {
     constexpr int EB = 6;
+    if (lazy) {
+        if (inv) return nullptr;
+        switch (n * 2) {
+        case 20: return fused_kernel<10, false, true, EB>;
+        case 22: return fused_kernel<11, false, true, EB>;
+        case 24: return fused_kernel<12, false, true, EB>;
+        case 26: return fused_kernel<13, false, true, EB>;
+        case 28: return fused_kernel<14, false, true, EB>;
+        case 30: return fused_kernel<15, false, true, EB>;
+        case 32: return fused_kernel<16, false, true, EB>;
+            default: return nullptr;
+        }
+    }
     switch (n * 2 + (inv ? 1 : 0)) {
         case 20: return fused_kernel<10, false, false, EB>;
         case 21: return fused_kernel<10, true, false, EB>;

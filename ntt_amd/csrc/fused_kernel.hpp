@@ -250,7 +250,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         }
         if (!INV && (c.flags & 1)) {
 #pragma unroll
-            for (int x = 0; x < E; ++x) v[x] = LAZY ? mont_mul_lazy(v[x], c.scale, c.q, c.q_inv) : mont_mul(v[x], c.scale, c.q, c.q_inv);
+            for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], c.scale, c.q, c.q_inv);
         }
     } else {
 #pragma unroll

@@ -8,8 +8,11 @@ namespace ntt {
 
 // LDS of the fused kernel: half of the workgroup's 64K words plus padding.
 constexpr uint32_t FUSED_LDS_WORDS = 32768 + 1024;
-// Default window width (7: 512 threads x 128 words; 6: 1024 threads x 64 words).
-constexpr int FUSED_DEFAULT_EB = 7;
+// Default window widths (7: 512 threads x 128 words; 6: 1024 threads x 64
+// words) per direction, the measured faster ones at N = 2^16 on MI355X; the
+// autotuner overrides them per launch size.
+constexpr int FUSED_DEFAULT_EB_FWD = 7;
+constexpr int FUSED_DEFAULT_EB_INV = 6;
 
 struct FusedParams;
 using FusedFn = void (*)(FusedParams);

@@ -145,6 +145,25 @@ int plan_engine(const ntt_plan* P, size_t count) {
     return fused_supported(P->logN) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
 }
 
+void fused_variant_for(const ntt_context* ctx, int logN, size_t total_limbs, bool inverse, bool lazy_capable,
+                       int* eb, bool* lazy) {
+    TunedEntry t;
+    const bool tuned = ctx->tuned_lookup(logN, total_limbs, &t);
+    *eb = tuned ? (inverse ? t.fused_eb_inv : t.fused_eb_fwd) : (inverse ? FUSED_DEFAULT_EB_INV : FUSED_DEFAULT_EB_FWD);
+    *lazy = (tuned ? t.fused_lazy : true) && lazy_capable && !inverse;
+}
+
+void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, int* eb, bool* lazy, int* stagger) {
+    TunedEntry t;
+    const bool tuned = !P->cfg_fixed && P->ctx->tuned_lookup(P->logN, P->limbs * count, &t);
+    *eb = P->fused_eb ? P->fused_eb
+                      : (tuned ? (inverse ? t.fused_eb_inv : t.fused_eb_fwd)
+                               : (inverse ? FUSED_DEFAULT_EB_INV : FUSED_DEFAULT_EB_FWD));
+    const bool want_lazy = P->fused_arith >= 0 ? P->fused_arith == 1 : (tuned ? t.fused_lazy : true);
+    *lazy = want_lazy && P->lazy && !inverse;   // lazy kernels: forward only, every q < 2^30
+    *stagger = tuned ? t.stagger : 0;
+}
+
 int log2_bucket(size_t v) {
     int b = 0;
     while ((size_t(2) << b) <= v) ++b;
@@ -225,10 +244,10 @@ int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out) {
 static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse, void* stream) {
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
     if (plan_engine(plan, count) == NTT_ENGINE_FUSED) {
-        TunedEntry t;
-        const bool tuned = !plan->cfg_fixed && plan->ctx->tuned_lookup(plan->logN, plan->limbs * count, &t);
-        const int eb = ntt_plan_get_fused_shape(plan, count);
-        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, eb, tuned ? t.stagger : 0);
+        int eb = 0, stagger = 0;
+        bool lazy = false;
+        plan_fused_variant(plan, count, inverse, &eb, &lazy, &stagger);
+        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, eb, lazy, stagger);
     }
     return launch_transform(plan_launch(plan, A_dev, count, inverse), (hipStream_t)stream);
 }
@@ -257,12 +276,20 @@ int ntt_plan_set_fused_shape(ntt_plan* plan, int eb) {
     return NTT_OK;
 }
 
-int ntt_plan_get_fused_shape(const ntt_plan* plan, size_t count) {
-    if (!plan) return -NTT_ERR_INVALID_ARGUMENT;
-    if (plan->fused_eb) return plan->fused_eb;
-    TunedEntry t;
-    const bool tuned = !plan->cfg_fixed && plan->ctx->tuned_lookup(plan->logN, plan->limbs * count, &t);
-    return tuned ? t.fused_eb : FUSED_DEFAULT_EB;
+int ntt_plan_set_fused_arith(ntt_plan* plan, int lazy) {
+    if (!plan || lazy < -1 || lazy > 1) return NTT_ERR_INVALID_ARGUMENT;
+    plan->fused_arith = lazy;
+    return NTT_OK;
+}
+
+int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction, int* eb, int* lazy) {
+    if (!plan || (direction != NTT_FORWARD && direction != NTT_INVERSE)) return NTT_ERR_INVALID_ARGUMENT;
+    int e = 0, st = 0;
+    bool lz = false;
+    plan_fused_variant(plan, count, direction == NTT_INVERSE, &e, &lz, &st);
+    if (eb) *eb = e;
+    if (lazy) *lazy = lz ? 1 : 0;
+    return NTT_OK;
 }
 
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count) {

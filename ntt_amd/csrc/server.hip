@@ -37,7 +37,7 @@
 #include "launch.hpp"
 
 namespace ntt {
-int launch_fused(int n, bool inv, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
+int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
 int fused_prepare();
 uint32_t fused_items(int logN, size_t limbs);
 }  // namespace ntt
@@ -210,13 +210,14 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
         if (rc) return rc;
     }
     // 2. group by (logN, direction)
-    std::map<std::pair<int, int>, std::vector<size_t>> groups;
-    for (size_t i = 0; i < reqs.size(); ++i) groups[{reqs[i]->r.logN, reqs[i]->r.direction}].push_back(i);
+    std::map<std::tuple<int, int, int>, std::vector<size_t>> groups;   // (logN, direction, lazy-capable)
+    for (size_t i = 0; i < reqs.size(); ++i)
+        groups[std::make_tuple(reqs[i]->r.logN, reqs[i]->r.direction, reqs[i]->plan->lazy ? 1 : 0)].push_back(i);
     size_t nd = 0;
-    std::vector<std::tuple<int, bool, size_t, size_t, uint32_t>> fl;   // n, inv, first desc, count, items
+    std::vector<std::tuple<int, bool, size_t, size_t, uint32_t, bool>> fl;   // n, inv, first, count, items, lazy-capable
     for (auto& kv : groups) {
-        const int n = kv.first.first;
-        const bool inv = kv.first.second == NTT_INVERSE;
+        const int n = std::get<0>(kv.first);
+        const bool inv = std::get<1>(kv.first) == NTT_INVERSE;
         if (fused_supported(n)) {
             const size_t first = nd;
             uint64_t items = 0;
@@ -234,7 +235,7 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
                 j.inverse = inv ? 1 : 0;
                 items += fused_items(n, q.r.limbs);
             }
-            fl.emplace_back(n, inv, first, kv.second.size(), (uint32_t)items);
+            fl.emplace_back(n, inv, first, kv.second.size(), (uint32_t)items, std::get<2>(kv.first) != 0);
         } else {
             for (size_t i : kv.second) {
                 const Req& q = *reqs[i];
@@ -255,11 +256,12 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
         p.njobs = (uint32_t)std::get<3>(g);
         p.total_items = std::get<4>(g);
         const int n = std::get<0>(g);
-        TunedEntry t;
         size_t tot = 0;
         for (size_t k = 0; k < std::get<3>(g); ++k) tot += s.h_desc[std::get<2>(g) + k].limbs;
-        const int eb = S->ctx->tuned_lookup(n, tot, &t) ? t.fused_eb : FUSED_DEFAULT_EB;
-        if ((rc = launch_fused(n, std::get<1>(g), eb, p, s.stream, 0))) return rc;
+        int eb = 0;
+        bool lazy = false;
+        fused_variant_for(S->ctx, n, tot, std::get<1>(g), std::get<5>(g), &eb, &lazy);
+        if ((rc = launch_fused(n, std::get<1>(g), eb, lazy, p, s.stream, 0))) return rc;
         ++*fused;
     }
     return ck(hipEventRecord(s.done, s.stream), "hipEventRecord(slot)");

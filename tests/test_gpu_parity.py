@@ -248,10 +248,11 @@ def test_autotune_result_is_valid_and_exact(env, tmp_path):
 
 
 @pytest.mark.parametrize("logN", [10, 11, 12, 13, 14, 15, 16])
-@pytest.mark.parametrize("engine,eb", [(2, 7), (2, 6), (1, 0)])
-def test_engines_all_sizes(env, logN, engine, eb):
-    """Fused limb kernel (both workgroup shapes) and two-pass kernels,
-    bit-exact; limb counts that leave the last fused work item partly empty."""
+@pytest.mark.parametrize("engine,eb,arith", [(2, 7, 1), (2, 7, 0), (2, 6, 1), (2, 6, 0), (1, 0, -1)])
+def test_engines_all_sizes(env, logN, engine, eb, arith):
+    """Fused limb kernel (both workgroup shapes; lazy and canonical forward
+    arithmetic) and two-pass kernels, bit-exact; limb counts that leave the
+    last fused work item partly empty."""
     ntt_amd, gu, orc = env
     lpw = 1 << (16 - logN)
     for limbs in sorted({1, 3, lpw + 1}):
@@ -261,10 +262,13 @@ def test_engines_all_sizes(env, logN, engine, eb):
             plan = ctx.plan(logN, moduli, domain)
             plan.set_engine(engine)
             plan.set_fused_shape(eb)
+            plan.set_fused_arith(arith)
             assert plan.engine(1) == engine
+            if engine == 2:
+                assert plan.fused_variant(1, 0) == (eb, arith == 1)
             x = gu.random_operand(logN, moduli, 1000 + limbs)
             A = gu.to_dev(x)
-            _print("case engine", engine, eb, logN, limbs, domain)
+            _print("case engine", engine, eb, arith, logN, limbs, domain)
             plan.forward(A)
             gu.sync()
             assert np.array_equal(gu.to_host(A).reshape(x.shape), orc.forward(x, logN, moduli, domain))
