@@ -190,6 +190,15 @@ int ntt_plan_destroy(ntt_plan* plan);
 int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out);
 /* Selects the kernel family (ntt_engine); NTT_ERR_CONFIG if unsupported for logN. */
 int ntt_plan_set_engine(ntt_plan* plan, int engine);
+/* Fused per-limb constant multiplies (the elementwise prologue/epilogue of
+ * Cheddar's NTT handlers, NTT_cheddar.cu:128-229 and MultConstLazy at
+ * 286-293): after this call the plan's forward transform computes
+ * NTT(prologue[l] * x) and its inverse computes epilogue[l] * INTT(x), limb by
+ * limb, at no extra pass (the constants fold into the R^2 / N^-1 multiplies
+ * the kernels already do on load and store).  Constants are host arrays of
+ * `limbs` words, reduced mod q; NULL means 1.  Synchronous; do not call while
+ * launches of this plan are in flight. */
+int ntt_plan_set_constants(ntt_plan* plan, const uint32_t* prologue, const uint32_t* epilogue);
 /* Workgroup shape of the fused engine for both directions: 7 = 512 threads x
  * 128 words each, 6 = 1024 threads x 64 words, 0 = the tuned choice
  * (default: 7 forward, 6 inverse). */

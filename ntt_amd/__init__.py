@@ -209,6 +209,7 @@ _PROTOS = {
     "ntt_plan_set_engine": (_I, [_P, _I]),
     "ntt_plan_set_fused_shape": (_I, [_P, _I]),
     "ntt_plan_set_fused_arith": (_I, [_P, _I]),
+    "ntt_plan_set_constants": (_I, [_P, _PU32, _PU32]),
     "ntt_plan_get_fused_variant": (_I, [_P, _SZ, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ntt_plan_get_engine": (_I, [_P, _SZ]),
     "ntt_autotune": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _SZ, _I, _I, ctypes.POINTER(TuneResult)]),
@@ -644,6 +645,18 @@ class Plan:
         _check(_lib.ntt_plan_get_fused_variant(self.handle, count, direction, ctypes.byref(eb), ctypes.byref(lz)),
                "ntt_plan_get_fused_variant")
         return eb.value, bool(lz.value)
+
+    def set_constants(self, prologue=None, epilogue=None):
+        """Fused per-limb constant multiplies: forward computes NTT(prologue * x),
+        inverse computes epilogue * INTT(x) (None = 1)."""
+        def arr(v):
+            if v is None:
+                return None
+            a = (ctypes.c_uint32 * self.limbs)(*[int(x) for x in v])
+            return ctypes.cast(a, _PU32), a
+        p, e = arr(prologue), arr(epilogue)
+        _check(_lib.ntt_plan_set_constants(self.handle, p[0] if p else None, e[0] if e else None),
+               "ntt_plan_set_constants")
 
     def set_fused_arith(self, lazy: int):
         """Fused forward arithmetic: 1 lazy (when every q < 2^30), 0 canonical, -1 tuned."""

@@ -7,6 +7,7 @@
 #include <mutex>
 #include <tuple>
 #include <utility>
+#include <vector>
 
 #include "../../include/ntt_amd.h"
 #include "launch.hpp"
@@ -68,6 +69,9 @@ struct ntt_plan {
     uint32_t* d_scale_inv = nullptr;               // Montgomery multiplier on the inverse output
     const ntt::LimbConst* d_lc_fwd = nullptr;      // per-limb constants for the fused kernel
     const ntt::LimbConst* d_lc_inv = nullptr;
+    // host copies for rebuilding the scale words (ntt_plan_set_constants)
+    std::vector<uint32_t> h_q, h_q_inv, h_r2, h_degree_inv;
+    bool prologue = false;                         // forward input multiplied by per-limb constants
 };
 
 namespace ntt {

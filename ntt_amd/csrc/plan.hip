@@ -80,6 +80,16 @@ int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, siz
     }
     P->lazy = true;
     for (size_t l = 0; l < limbs; ++l) P->lazy = P->lazy && q[l] < (1u << 30);
+    P->h_q = q;
+    P->h_q_inv = qi;
+    P->h_r2.resize(limbs);
+    P->h_degree_inv.resize(limbs);
+    for (size_t l = 0; l < limbs; ++l) {
+        ntt_modulus_runtime rt;
+        make_runtime(logN, moduli[l].qi, moduli[l].primitive_root, &rt);
+        P->h_r2[l] = rt.r2;
+        P->h_degree_inv[l] = rt.degree_inv;
+    }
     const size_t u32b = limbs * sizeof(uint32_t), ptrb = limbs * sizeof(void*);
     const size_t lcb = limbs * sizeof(LimbConst);
     if (hipMalloc(&P->d_const, 2 * lcb + 4 * u32b + 2 * ptrb) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
@@ -119,7 +129,7 @@ TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool i
     L.q_inv = P->d_q_inv;
     L.scale = inverse ? P->d_scale_inv : P->d_scale_fwd;
     // The plain forward transform needs no input multiply.
-    L.scale_mode = (!inverse && P->domain == NTT_DOMAIN_PLAIN) ? SCALE_NONE : SCALE_AS_IS;
+    L.scale_mode = (!inverse && P->domain == NTT_DOMAIN_PLAIN && !P->prologue) ? SCALE_NONE : SCALE_AS_IS;
     L.logN = P->logN;
     L.limbs = P->limbs;
     L.count = count;
@@ -273,6 +283,34 @@ int ntt_plan_set_fused_shape(ntt_plan* plan, int eb) {
     if (!plan) return NTT_ERR_INVALID_ARGUMENT;
     if (eb != 0 && eb != 6 && eb != 7) return NTT_ERR_CONFIG;
     plan->fused_eb = eb;
+    return NTT_OK;
+}
+
+int ntt_plan_set_constants(ntt_plan* plan, const uint32_t* prologue, const uint32_t* epilogue) {
+    if (!plan) return NTT_ERR_INVALID_ARGUMENT;
+    const size_t L = plan->limbs;
+    const bool mont = plan->domain == NTT_DOMAIN_MONTGOMERY;
+    std::vector<uint32_t> sf(L), si(L);
+    std::vector<LimbConst> lcf(L), lci(L);
+    for (size_t l = 0; l < L; ++l) {
+        const uint64_t q = plan->h_q[l];
+        const uint64_t c = prologue ? prologue[l] % q : 1, e = epilogue ? epilogue[l] % q : 1;
+        // forward: mont_mul(x, K) = x K / R must be x c R (Montgomery) or x c (plain)
+        sf[l] = mont ? (uint32_t)(c * plan->h_r2[l] % q) : to_mont_host((uint32_t)c, (uint32_t)q);
+        // inverse: mont_mul(v, K) must be v e N^-1 / R (Montgomery input) or v e N^-1 (plain)
+        const uint32_t de = (uint32_t)((uint64_t)plan->h_degree_inv[l] * e % q);
+        si[l] = mont ? de : to_mont_host(de, (uint32_t)q);
+        lcf[l] = {(uint32_t)q, plan->h_q_inv[l], sf[l], (mont || prologue) ? 1u : 0u};
+        lci[l] = {(uint32_t)q, plan->h_q_inv[l], si[l], 1u};
+    }
+    hipError_t err = hipMemcpy(plan->d_scale_fwd, sf.data(), L * 4, hipMemcpyHostToDevice);
+    if (err == hipSuccess) err = hipMemcpy(plan->d_scale_inv, si.data(), L * 4, hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(const_cast<LimbConst*>(plan->d_lc_fwd), lcf.data(), L * sizeof(LimbConst), hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+        err = hipMemcpy(const_cast<LimbConst*>(plan->d_lc_inv), lci.data(), L * sizeof(LimbConst), hipMemcpyHostToDevice);
+    if (err != hipSuccess) return record_hip_error(err, "hipMemcpy(plan constants)");
+    plan->prologue = prologue != nullptr;
     return NTT_OK;
 }
 

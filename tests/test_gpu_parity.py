@@ -319,3 +319,41 @@ def test_engines_canonical_arithmetic(env, engine):
         plan.inverse(A)
         gu.sync()
         assert np.array_equal(gu.to_host(A).reshape(x.shape), x)
+
+
+@pytest.mark.parametrize("logN,engine", [(12, 2), (12, 1), (16, 2), (16, 1), (17, 1)])
+@pytest.mark.parametrize("domain", [1, 0])
+def test_plan_constants_prologue_epilogue(env, logN, engine, domain):
+    """Fused per-limb constant multiplies: forward = NTT(c * x), inverse =
+    e * INTT(y), bit-exact against the oracle applied to the scaled operand."""
+    ntt_amd, gu, orc = env
+    limbs = 5
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    qs = np.array([q for q, _ in moduli], np.uint64)[:, None]
+    rng = np.random.default_rng(logN * 10 + engine + domain)
+    c = [int(rng.integers(0, q)) for q, _ in moduli]
+    e = [int(rng.integers(0, 1 << 32)) for _ in moduli]      # reduced mod q by the library
+    x = gu.random_operand(logN, moduli, 77)
+    ctx = ntt_amd.Context(0)
+    plan = ctx.plan(logN, moduli, domain)
+    plan.set_engine(engine)
+    plan.set_constants(c, e)
+    A = gu.to_dev(x)
+    _print("case constants", logN, engine, domain)
+    plan.forward(A)
+    gu.sync()
+    xs = (x.astype(np.uint64) * np.array(c, np.uint64)[:, None] % qs).astype(np.uint32)
+    fwd = gu.to_host(A).reshape(x.shape)
+    assert np.array_equal(fwd, orc.forward(xs, logN, moduli, domain))
+    plan.inverse(A)
+    gu.sync()
+    ev = np.array([v % int(q) for v, (q, _) in zip(e, moduli)], np.uint64)[:, None]
+    exp = (orc.inverse(fwd, logN, moduli, domain).astype(np.uint64) * ev % qs).astype(np.uint32)
+    assert np.array_equal(gu.to_host(A).reshape(x.shape), exp)
+    # back to plain transforms
+    plan.set_constants(None, None)
+    A.upload(x)
+    plan.forward(A)
+    plan.inverse(A)
+    gu.sync()
+    assert np.array_equal(gu.to_host(A).reshape(x.shape), x)
