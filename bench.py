@@ -95,15 +95,12 @@ def bench_mixed(ntt_amd, dev, stream, reps: int = 10, tune: bool = True, seed: i
     ctx = ntt_amd.Context(0)
     t0 = time.perf_counter()
     if tune:
-        # tune each size at the launch it will see: one fused group per
-        # (logN, direction) for logN <= 16, single jobs for logN = 17
+        # tune each size at the launch it will see: one launch per
+        # (logN, direction) group over all its jobs' limbs
         for n in sorted({j.logN for j in wl}):
             tot = max(sum(j.limbs for j in wl if j.logN == n and j.direction == d) for d in (0, 1))
-            if n <= 16:
-                limbs = min(tot, 48)
-                ctx.autotune(n, workloads.moduli_for(n, limbs), count=max(1, tot // limbs), reps=2)
-            else:
-                ctx.autotune(n, workloads.moduli_for(n, 24), count=1, reps=2)
+            limbs = min(tot, 48)
+            ctx.autotune(n, workloads.moduli_for(n, limbs), count=max(1, tot // limbs), reps=2)
     tune_s = time.perf_counter() - t0
     bufs, spec = [], []
     qcache = {}
@@ -168,7 +165,7 @@ def bench_mixed(ntt_amd, dev, stream, reps: int = 10, tune: bool = True, seed: i
 
 
 def bench_served(ntt_amd, requests: int = 20000, fractions=(0.25, 0.5, 0.75, 0.9, 1.0, 1.1),
-                 headline: float = 0.75, seed: int = 0, budget_gb: float = 64.0):
+                 headline: float = 0.75, seed: int = 0, budget_gb: float = 64.0, tune: bool = True):
     """The served stream: `requests` requests, logN ~ U{13..17}, L log-uniform
     on [4, 64], direction ~ Bernoulli(1/2), Montgomery, seeded operands
     generated on the device at admission, 64 GB operand budget.  First the
@@ -178,6 +175,10 @@ def bench_served(ntt_amd, requests: int = 20000, fractions=(0.25, 0.5, 0.75, 0.9
     from ntt_amd import workloads
 
     ctx = ntt_amd.Context(0)
+    if tune:
+        # launches near saturation hold a few hundred limbs per (logN, direction)
+        for n in range(13, 18):
+            ctx.autotune(n, workloads.moduli_for(n, 48), count=8, reps=2)
     srv = ctx.server(hbm_budget_bytes=int(budget_gb * (1 << 30)), slots=4, max_jobs_per_launch=256)
     base = workloads.serving_stream(seed, requests, rate_per_s=1.0)
     burst = [workloads.Request(0.0, r.job, r.seed) for r in base]
@@ -450,7 +451,7 @@ def main():
 
     served = None
     if not args.no_served:
-        served = bench_served(ntt_amd, requests=args.served_requests)
+        served = bench_served(ntt_amd, requests=args.served_requests, tune=not args.no_tune)
         log("served: " + json.dumps({k: v for k, v in served.items() if k != "sweep"}))
 
     cpu = cpu1 = None

@@ -263,7 +263,9 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path);
  * whose main() runs one fixed-size transform at a time
  * (src/v04_montgomery.cu:969-1062).  Jobs of one (logN, direction) with
  * 10 <= logN <= 16 run as ONE persistent fused kernel that walks a
- * device-resident job table; other sizes run as two-pass launches per job.
+ * device-resident job table; jobs of other sizes run, per (logN, direction,
+ * domain), as ONE two-pass transform over all their limbs through
+ * device-resident per-limb operand pointers.
  * Groups are spread over forked streams so they overlap on the device.  The
  * whole launch sequence can be captured once into a hipGraph and replayed.
  * Twiddle tables come from the context's cache (shared across jobs).
@@ -289,10 +291,10 @@ int ntt_batch_run(ntt_batch* batch, void* stream);
 int ntt_batch_capture(ntt_batch* batch, void* stream);
 /* Launches the captured graph on `stream` (captures first if needed). */
 int ntt_batch_replay(ntt_batch* batch, void* stream);
-/* Points job `index` at another operand of the same shape.  Fused-group jobs
- * read their operand address from the device job table, so a captured graph
- * stays valid; for a two-pass job the graph is dropped and re-captured by the
- * next replay.  Synchronous with respect to the device. */
+/* Points job `index` at another operand of the same shape.  Every job reads
+ * its operand address from device memory (the fused job table or the per-limb
+ * pointer array), so a captured graph stays valid.  Synchronous with respect
+ * to the device. */
 int ntt_batch_set_data(ntt_batch* batch, size_t index, uint32_t* data);
 /* fused_groups: persistent fused launches; two_pass_jobs: jobs on two-pass
  * kernels; streams: concurrent streams used. */

@@ -4,7 +4,8 @@
 // becomes one persistent fused launch over a device-resident table of
 // FusedJob descriptors (one per job, each with its own limb count, table
 // pointers and per-limb constants), so any number of jobs of that size costs
-// one kernel.  Jobs of other sizes run as two-pass launches.  Work units are
+// one kernel.  Jobs of other sizes run, per (logN, direction, domain), as one
+// two-pass transform over all their limbs (per-limb operand pointers).  Work units are
 // balanced over the caller's stream plus up to three streams owned by the
 // batch (fork/join through events), so groups overlap on the device.  The
 // same enqueue sequence is captured into a hipGraph for replay; it contains
@@ -23,6 +24,7 @@
 #include "fused_types.hpp"
 #include "host_math.hpp"
 #include "launch.hpp"
+#include "phase_kernels.hpp"
 
 namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
@@ -51,9 +53,22 @@ struct FusedGroup {
     double work = 0;                // butterflies, for stream balancing
 };
 
-struct Unit {                       // one enqueue: a fused group or a two-pass job
+// Jobs of one (logN, direction, domain, arithmetic) that the fused kernel
+// does not cover run as ONE two-pass transform over all their limbs, through
+// per-limb operand / table / constant arrays.
+struct TwoPassGroup {
+    int n = 0;
+    bool inverse = false;
+    int domain = NTT_DOMAIN_MONTGOMERY;
+    bool lazy = false;
+    size_t limb_begin = 0;          // first entry in the batch's per-limb arrays
+    size_t limbs = 0;
+    double work = 0;
+};
+
+struct Unit {                       // one enqueue: a fused group (group >= 0) or a two-pass group
     int group = -1;
-    size_t job = 0;
+    int tp_group = -1;
     double work = 0;
 };
 
@@ -69,6 +84,15 @@ struct ntt_batch {
     std::vector<FusedGroup> groups;
     std::vector<FusedJob> desc;                 // host copy of the device job table
     FusedJob* d_desc = nullptr;
+    std::vector<TwoPassGroup> tp_groups;
+    std::vector<size_t> job_limb0;              // two-pass jobs: first entry in the per-limb arrays
+    std::vector<uint32_t*> h_A;                 // per-limb operand pointers (host copy)
+    void* d_tp = nullptr;                       // per-limb arrays: A, tw, q, q_inv, scale
+    uint32_t* const* d_A = nullptr;
+    const uint32_t* const* d_tw = nullptr;
+    const uint32_t* d_q = nullptr;
+    const uint32_t* d_qi = nullptr;
+    const uint32_t* d_scale = nullptr;
     std::vector<std::vector<Unit>> lanes;       // lane 0 = caller's stream
     hipStream_t aux[BATCH_AUX_STREAMS] = {};
     hipEvent_t fork = nullptr;
@@ -85,6 +109,7 @@ struct ntt_batch {
             if (e) (void)hipEventDestroy(e);
         if (fork) (void)hipEventDestroy(fork);
         if (d_desc) (void)hipFree(d_desc);
+        if (d_tp) (void)hipFree(d_tp);
     }
 };
 
@@ -113,8 +138,24 @@ int enqueue(ntt_batch* b, hipStream_t stream) {
                 p.stagger = 0;
                 rc = launch_fused(g.n, g.inverse, g.eb, g.lazy, p, s, 0);
             } else {
-                const ntt_batch_job& j = b->jobs[u.job];
-                rc = launch_transform(plan_launch(b->job_plan[u.job], j.data, 1, j.direction == NTT_INVERSE), s);
+                const TwoPassGroup& g = b->tp_groups[(size_t)u.tp_group];
+                TransformLaunch L{};
+                L.A_ptrs = b->d_A + g.limb_begin;
+                L.tw_ptrs = b->d_tw + g.limb_begin;
+                L.moduli = b->d_q + g.limb_begin;
+                L.q_inv = b->d_qi + g.limb_begin;
+                L.scale = b->d_scale + g.limb_begin;
+                L.scale_mode = (!g.inverse && g.domain == NTT_DOMAIN_PLAIN) ? SCALE_NONE : SCALE_AS_IS;
+                L.logN = g.n;
+                L.limbs = g.limbs;
+                L.count = 1;
+                L.inverse = g.inverse;
+                L.lazy = g.lazy;
+                L.xcd_remap = 1;
+                TunedEntry t;
+                if (b->ctx->tuned_lookup(g.n, g.limbs, &t)) L.cfg = t.cfg;
+                else default_config(g.n, &L.cfg);
+                rc = launch_transform(L, s);
             }
             if (rc) break;
         }
@@ -152,7 +193,8 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
     b->job_desc.assign(njobs, 0);
     int rc = NTT_OK;
     std::map<std::tuple<int, int, int>, std::vector<size_t>> by_shape;   // (logN, inverse, lazy-capable) -> jobs
-    std::vector<size_t> two_pass;
+    std::map<std::tuple<int, int, int, int>, std::vector<size_t>> tp_shape;   // (logN, inverse, domain, lazy)
+    b->job_limb0.assign(njobs, 0);
     for (size_t i = 0; i < njobs; ++i) {
         const ntt_batch_job& j = jobs[i];
         if (!j.data || !j.moduli || j.limbs == 0 || j.logN < 1 || j.logN > NTT_MAX_LOG_N) return NTT_ERR_INVALID_ARGUMENT;
@@ -175,7 +217,9 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         b->jobs[i].moduli = nullptr;
         if (fused_supported(j.logN))
             by_shape[std::make_tuple(j.logN, j.direction == NTT_INVERSE ? 1 : 0, it->second->lazy ? 1 : 0)].push_back(i);
-        else two_pass.push_back(i);
+        else
+            tp_shape[std::make_tuple(j.logN, j.direction == NTT_INVERSE ? 1 : 0, j.domain, it->second->lazy ? 1 : 0)]
+                .push_back(i);
     }
 
     // descriptor table, one contiguous run per group
@@ -211,12 +255,35 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         }
         g.items = (uint32_t)items;
         fused_variant_for(ctx, g.n, total_limbs, g.inverse, lazy_capable, &g.eb, &g.lazy);
-        units.push_back({(int)b->groups.size(), 0, g.work});
+        units.push_back({(int)b->groups.size(), -1, g.work});
         b->groups.push_back(g);
     }
-    for (size_t i : two_pass) {
-        const ntt_batch_job& j = b->jobs[i];
-        units.push_back({-1, i, 0.5 * (double)((size_t)1 << j.logN) * j.logN * j.limbs});
+    std::vector<const uint32_t*> h_tw;
+    std::vector<uint32_t> h_q, h_qi, h_scale;
+    for (auto& kv : tp_shape) {
+        TwoPassGroup g;
+        g.n = std::get<0>(kv.first);
+        g.inverse = std::get<1>(kv.first) != 0;
+        g.domain = std::get<2>(kv.first);
+        g.lazy = std::get<3>(kv.first) != 0;
+        g.limb_begin = b->h_A.size();
+        for (size_t i : kv.second) {
+            const ntt_batch_job& j = b->jobs[i];
+            const ntt_plan* P = b->job_plan[i];
+            b->job_limb0[i] = b->h_A.size();
+            for (uint32_t l = 0; l < j.limbs; ++l) {
+                b->h_A.push_back(j.data + ((size_t)l << g.n));
+                h_tw.push_back(g.inverse ? P->h_tw_inv[l] : P->h_tw_fwd[l]);
+                h_q.push_back(P->h_q[l]);
+                h_qi.push_back(P->h_q_inv[l]);
+                h_scale.push_back(g.inverse ? P->h_scale_inv[l] : P->h_scale_fwd[l]);
+            }
+            g.work += 0.5 * (double)((size_t)1 << g.n) * g.n * j.limbs;
+        }
+        g.limbs = b->h_A.size() - g.limb_begin;
+        if (g.limbs > 0xffffffu) return NTT_ERR_INVALID_ARGUMENT;
+        units.push_back({-1, (int)b->tp_groups.size(), g.work});
+        b->tp_groups.push_back(g);
     }
 
     // longest-processing-time-first over the lanes
@@ -236,6 +303,24 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
     if ((rc = ck(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate"))) return rc;
 
     if (!b->groups.empty() && (rc = fused_prepare())) return rc;
+    if (!b->h_A.empty()) {
+        const size_t T = b->h_A.size();
+        const size_t bytes = T * (2 * sizeof(void*) + 3 * sizeof(uint32_t));
+        if (hipMalloc(&b->d_tp, bytes) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
+        std::vector<char> h(bytes);
+        char* base = reinterpret_cast<char*>(b->d_tp);
+        std::memcpy(h.data(), b->h_A.data(), T * sizeof(void*));
+        std::memcpy(h.data() + T * sizeof(void*), h_tw.data(), T * sizeof(void*));
+        std::memcpy(h.data() + 2 * T * sizeof(void*), h_q.data(), T * 4);
+        std::memcpy(h.data() + 2 * T * sizeof(void*) + T * 4, h_qi.data(), T * 4);
+        std::memcpy(h.data() + 2 * T * sizeof(void*) + 2 * T * 4, h_scale.data(), T * 4);
+        b->d_A = reinterpret_cast<uint32_t* const*>(base);
+        b->d_tw = reinterpret_cast<const uint32_t* const*>(base + T * sizeof(void*));
+        b->d_q = reinterpret_cast<const uint32_t*>(base + 2 * T * sizeof(void*));
+        b->d_qi = b->d_q + T;
+        b->d_scale = b->d_q + 2 * T;
+        if ((rc = ck(hipMemcpy(b->d_tp, h.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(limbs)"))) return rc;
+    }
     if (!b->desc.empty()) {
         const size_t bytes = b->desc.size() * sizeof(FusedJob);
         if (hipMalloc(&b->d_desc, bytes) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
@@ -289,15 +374,20 @@ int ntt_batch_set_data(ntt_batch* batch, size_t index, uint32_t* data) {
     if (!batch || index >= batch->jobs.size() || !data || (reinterpret_cast<uintptr_t>(data) & 15))
         return NTT_ERR_INVALID_ARGUMENT;
     batch->jobs[index].data = data;
-    if (batch->job_group[index] < 0) {
-        drop_graph(batch);
-        return NTT_OK;
-    }
-    const size_t d = batch->job_desc[index];
-    batch->desc[d].data = data;
     // the graph may still be running on another stream: copy after the device is idle
     int rc = ck(hipDeviceSynchronize(), "hipDeviceSynchronize");
     if (rc) return rc;
+    if (batch->job_group[index] < 0) {
+        // two-pass job: its limbs' entries in the per-limb operand array
+        const ntt_batch_job& j = batch->jobs[index];
+        const size_t l0 = batch->job_limb0[index];
+        for (uint32_t l = 0; l < j.limbs; ++l) batch->h_A[l0 + l] = data + ((size_t)l << j.logN);
+        return ck(hipMemcpy(const_cast<uint32_t**>(batch->d_A) + l0, batch->h_A.data() + l0, j.limbs * sizeof(void*),
+                            hipMemcpyHostToDevice),
+                  "hipMemcpy(limb pointers)");
+    }
+    const size_t d = batch->job_desc[index];
+    batch->desc[d].data = data;
     return ck(hipMemcpy(batch->d_desc + d, &batch->desc[d], sizeof(FusedJob), hipMemcpyHostToDevice),
               "hipMemcpy(job)");
 }
@@ -308,6 +398,7 @@ int ntt_batch_info(const ntt_batch* batch, int* fused_groups, int* two_pass_jobs
     for (int g : batch->job_group) tp += g < 0 ? 1 : 0;
     if (fused_groups) *fused_groups = (int)batch->groups.size();
     if (two_pass_jobs) *two_pass_jobs = tp;
+    (void)batch->tp_groups;
     if (streams) *streams = (int)batch->lanes.size();
     return NTT_OK;
 }

@@ -71,6 +71,8 @@ struct ntt_plan {
     const ntt::LimbConst* d_lc_inv = nullptr;
     // host copies for rebuilding the scale words (ntt_plan_set_constants)
     std::vector<uint32_t> h_q, h_q_inv, h_r2, h_degree_inv;
+    std::vector<const uint32_t*> h_tw_fwd, h_tw_inv;   // host copies of the table pointers
+    std::vector<uint32_t> h_scale_fwd, h_scale_inv;    // host copies of the scale words
     bool prologue = false;                         // forward input multiplied by per-limb constants
 };
 

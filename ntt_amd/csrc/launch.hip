@@ -96,7 +96,7 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, int mode,
 }
 
 int launch_transform(const TransformLaunch& L, hipStream_t stream) {
-    if (L.chunk == 0 || L.chunk >= L.count) return launch_transform_passes(L, stream, 3);
+    if (L.A_ptrs || L.chunk == 0 || L.chunk >= L.count) return launch_transform_passes(L, stream, 3);
     const size_t words = L.limbs << L.logN;
     for (size_t first = 0; first < L.count; first += L.chunk) {
         TransformLaunch c = L;
@@ -109,7 +109,9 @@ int launch_transform(const TransformLaunch& L, hipStream_t stream) {
 }
 
 int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pass_mask) {
-    if (!L.A || (!L.tw && !L.tw_ptrs) || !L.moduli || L.limbs == 0 || L.count == 0) return NTT_ERR_INVALID_ARGUMENT;
+    if ((!L.A && !L.A_ptrs) || (!L.tw && !L.tw_ptrs) || !L.moduli || L.limbs == 0 || L.count == 0)
+        return NTT_ERR_INVALID_ARGUMENT;
+    if (L.A_ptrs && (L.count != 1 || !L.tw_ptrs)) return NTT_ERR_INVALID_ARGUMENT;
     if (L.logN < 1 || L.logN > NTT_MAX_LOG_N) return NTT_ERR_INVALID_ARGUMENT;
     if (L.scale_mode != SCALE_NONE && !L.scale) return NTT_ERR_INVALID_ARGUMENT;
     if (reinterpret_cast<uintptr_t>(L.A) & 15) return NTT_ERR_INVALID_ARGUMENT;   // 16 B vector access
@@ -118,6 +120,7 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
 
     PhaseParams p{};
     p.A = L.A;
+    p.A_ptrs = L.A_ptrs;
     p.tw = L.tw;
     p.tw_ptrs = L.tw_ptrs;
     p.moduli = L.moduli;

@@ -43,7 +43,8 @@ enum ScaleMode : int {
 };
 
 struct PhaseParams {
-    uint32_t* A;              // operand base, operand k at A + k * limbs * N
+    uint32_t* A;              // operand base, operand k at A + k * limbs * N ...
+    uint32_t* const* A_ptrs;  // ... or per-limb operand pointers (count == 1)
     const uint32_t* tw;       // twiddle table, limb l at tw + l * N ...
     const uint32_t* const* tw_ptrs;  // ... or per-limb table pointers (twiddle cache)
     const uint32_t* moduli;   // q per limb
@@ -128,7 +129,7 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     const uint32_t op = rest % p.count;
     const uint32_t li = rest / p.count;
     const size_t N = (size_t)1 << n;
-    uint32_t* A = p.A + ((size_t)op * p.limbs + li) * N;
+    uint32_t* A = p.A_ptrs ? p.A_ptrs[li] : p.A + ((size_t)op * p.limbs + li) * N;
     const uint32_t* tw = p.tw_ptrs ? p.tw_ptrs[li] : p.tw + (size_t)li * N;
 
     const uint32_t q = p.moduli[li];

@@ -82,6 +82,10 @@ int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, siz
     for (size_t l = 0; l < limbs; ++l) P->lazy = P->lazy && q[l] < (1u << 30);
     P->h_q = q;
     P->h_q_inv = qi;
+    P->h_tw_fwd = twf;
+    P->h_tw_inv = twi;
+    P->h_scale_fwd = sf;
+    P->h_scale_inv = si;
     P->h_r2.resize(limbs);
     P->h_degree_inv.resize(limbs);
     for (size_t l = 0; l < limbs; ++l) {
@@ -311,6 +315,8 @@ int ntt_plan_set_constants(ntt_plan* plan, const uint32_t* prologue, const uint3
         err = hipMemcpy(const_cast<LimbConst*>(plan->d_lc_inv), lci.data(), L * sizeof(LimbConst), hipMemcpyHostToDevice);
     if (err != hipSuccess) return record_hip_error(err, "hipMemcpy(plan constants)");
     plan->prologue = prologue != nullptr;
+    plan->h_scale_fwd = sf;
+    plan->h_scale_inv = si;
     return NTT_OK;
 }
 

@@ -35,6 +35,7 @@
 #include "fused_types.hpp"
 #include "host_math.hpp"
 #include "launch.hpp"
+#include "phase_kernels.hpp"
 
 namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
@@ -116,11 +117,18 @@ struct Slot {
     hipEvent_t done = nullptr;
     FusedJob* h_desc = nullptr;     // pinned staging
     FusedJob* d_desc = nullptr;
+    // per-limb arrays of the two-pass groups (operand, table, q, q_inv, scale),
+    // pinned staging and device copy, SLOT_LIMB_BYTES bytes per limb
+    char* h_limb = nullptr;
+    char* d_limb = nullptr;
+    size_t limb_cap = 0;            // limbs
     bool busy = false;
     std::vector<std::shared_ptr<Req>> reqs;
 };
 
 using PlanKey = std::tuple<int, int, std::vector<uint32_t>>;
+constexpr size_t SLOT_LIMB_BYTES = 2 * sizeof(void*) + 3 * sizeof(uint32_t);
+constexpr size_t SLOT_LIMBS_PER_JOB = 64;   // per-limb capacity of a slot = max jobs x this
 
 }  // namespace
 
@@ -153,6 +161,8 @@ struct ntt_server {
             if (s.done) (void)hipEventDestroy(s.done);
             if (s.h_desc) (void)hipHostFree(s.h_desc);
             if (s.d_desc) (void)hipFree(s.d_desc);
+            if (s.h_limb) (void)hipHostFree(s.h_limb);
+            if (s.d_limb) (void)hipFree(s.d_limb);
         }
         if (arena) (void)hipFree(arena);
     }
@@ -210,9 +220,12 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
         if (rc) return rc;
     }
     // 2. group by (logN, direction)
-    std::map<std::tuple<int, int, int>, std::vector<size_t>> groups;   // (logN, direction, lazy-capable)
+    // (logN, direction, lazy-capable, domain)
+    std::map<std::tuple<int, int, int, int>, std::vector<size_t>> groups;
     for (size_t i = 0; i < reqs.size(); ++i)
-        groups[std::make_tuple(reqs[i]->r.logN, reqs[i]->r.direction, reqs[i]->plan->lazy ? 1 : 0)].push_back(i);
+        groups[std::make_tuple(reqs[i]->r.logN, reqs[i]->r.direction, reqs[i]->plan->lazy ? 1 : 0, reqs[i]->r.domain)]
+            .push_back(i);
+    size_t used = 0;                 // bytes of the slot's per-limb buffers, 16-byte aligned regions
     size_t nd = 0;
     std::vector<std::tuple<int, bool, size_t, size_t, uint32_t, bool>> fl;   // n, inv, first, count, items, lazy-capable
     for (auto& kv : groups) {
@@ -237,12 +250,58 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
             }
             fl.emplace_back(n, inv, first, kv.second.size(), (uint32_t)items, std::get<2>(kv.first) != 0);
         } else {
+            size_t T = 0;
+            for (size_t i : kv.second) T += reqs[i]->r.limbs;
+            const size_t region = (T * SLOT_LIMB_BYTES + 15) & ~(size_t)15;
+            if (used + region > s.limb_cap * SLOT_LIMB_BYTES) {
+                // beyond the slot's per-limb capacity: one launch per request
+                for (size_t i : kv.second) {
+                    const Req& q = *reqs[i];
+                    rc = launch_transform(plan_launch(q.plan, S->arena + q.off / sizeof(uint32_t), 1, inv), s.stream);
+                    if (rc) return rc;
+                    ++*two_pass;
+                }
+                continue;
+            }
+            // one two-pass transform over every limb of the group
+            char* h = s.h_limb + used;
+            char* d = s.d_limb + used;
+            uint32_t** hA = reinterpret_cast<uint32_t**>(h);
+            const uint32_t** hT = reinterpret_cast<const uint32_t**>(h + T * sizeof(void*));
+            uint32_t* hq = reinterpret_cast<uint32_t*>(h + 2 * T * sizeof(void*));
+            size_t k = 0;
             for (size_t i : kv.second) {
                 const Req& q = *reqs[i];
-                rc = launch_transform(plan_launch(q.plan, S->arena + q.off / sizeof(uint32_t), 1, inv), s.stream);
-                if (rc) return rc;
-                ++*two_pass;
+                uint32_t* base = S->arena + q.off / sizeof(uint32_t);
+                for (uint32_t l = 0; l < q.r.limbs; ++l, ++k) {
+                    hA[k] = base + ((size_t)l << n);
+                    hT[k] = inv ? q.plan->h_tw_inv[l] : q.plan->h_tw_fwd[l];
+                    hq[k] = q.plan->h_q[l];
+                    hq[T + k] = q.plan->h_q_inv[l];
+                    hq[2 * T + k] = inv ? q.plan->h_scale_inv[l] : q.plan->h_scale_fwd[l];
+                }
             }
+            rc = ck(hipMemcpyAsync(d, h, T * SLOT_LIMB_BYTES, hipMemcpyHostToDevice, s.stream), "hipMemcpyAsync(limbs)");
+            if (rc) return rc;
+            used += region;
+            TransformLaunch L{};
+            L.A_ptrs = reinterpret_cast<uint32_t* const*>(d);
+            L.tw_ptrs = reinterpret_cast<const uint32_t* const*>(d + T * sizeof(void*));
+            L.moduli = reinterpret_cast<const uint32_t*>(d + 2 * T * sizeof(void*));
+            L.q_inv = L.moduli + T;
+            L.scale = L.moduli + 2 * T;
+            L.scale_mode = (!inv && std::get<3>(kv.first) == NTT_DOMAIN_PLAIN) ? SCALE_NONE : SCALE_AS_IS;
+            L.logN = n;
+            L.limbs = T;
+            L.count = 1;
+            L.inverse = inv;
+            L.lazy = std::get<2>(kv.first) != 0;
+            L.xcd_remap = 1;
+            TunedEntry t;
+            if (S->ctx->tuned_lookup(n, T, &t)) L.cfg = t.cfg;
+            else default_config(n, &L.cfg);
+            if ((rc = launch_transform(L, s.stream))) return rc;
+            ++*two_pass;
         }
     }
     if (nd) {
@@ -412,6 +471,14 @@ int ntt_server_create(ntt_context* ctx, const ntt_server_config* cfg, ntt_server
         if ((rc = ck(hipHostMalloc((void**)&s.h_desc, db, hipHostMallocDefault), "hipHostMalloc"))) return rc;
         if (hipMalloc(&s.d_desc, db) != hipSuccess) {
             s.d_desc = nullptr;
+            return NTT_ERR_OUT_OF_MEMORY;
+        }
+        s.limb_cap = (size_t)cfg->max_jobs_per_launch * SLOT_LIMBS_PER_JOB;
+        if ((rc = ck(hipHostMalloc((void**)&s.h_limb, s.limb_cap * SLOT_LIMB_BYTES, hipHostMallocDefault),
+                     "hipHostMalloc")))
+            return rc;
+        if (hipMalloc(&s.d_limb, s.limb_cap * SLOT_LIMB_BYTES) != hipSuccess) {
+            s.d_limb = nullptr;
             return NTT_ERR_OUT_OF_MEMORY;
         }
     }

@@ -100,8 +100,9 @@ def test_batch_graph_replays(env):
 
 
 def test_batch_set_data(env):
-    """Re-pointing a fused-group job keeps the graph; re-pointing a two-pass
-    job re-captures it; both give the oracle's result on the new operand."""
+    """Re-pointing a fused-group job and a two-pass job keeps the captured
+    graph (both read their operand addresses from device memory); replays
+    give the oracle's result on the new operands."""
     ntt_amd, gu, orc = env
     sizes = [13, 17, 13]
     jobs = _mixed_jobs(ntt_amd, gu, 11, sizes, 4)
