@@ -398,7 +398,6 @@ int ntt_batch_info(const ntt_batch* batch, int* fused_groups, int* two_pass_jobs
     for (int g : batch->job_group) tp += g < 0 ? 1 : 0;
     if (fused_groups) *fused_groups = (int)batch->groups.size();
     if (two_pass_jobs) *two_pass_jobs = tp;
-    (void)batch->tp_groups;
     if (streams) *streams = (int)batch->lanes.size();
     return NTT_OK;
 }

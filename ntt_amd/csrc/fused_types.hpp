@@ -34,7 +34,7 @@ struct FusedJob {
     uint32_t period;
     uint32_t item_begin;   // first work item of this job (prefix sum)
     uint16_t n;
-    uint16_t inverse;
+    uint16_t inverse;      // descriptive: the kernel's direction is its template parameter
 };
 
 struct FusedParams {
