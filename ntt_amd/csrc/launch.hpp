@@ -39,4 +39,17 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
 int record_hip_error(hipError_t e, const char* where);
 size_t device_max_lds();
 
+// One seeded fill (ntt_rt_fill_uniform semantics) of a batched fill launch.
+struct FillJob {
+    uint32_t* dst;
+    const uint32_t* moduli;    // q per limb (device)
+    uint64_t word_begin;       // prefix over the launch's jobs
+    uint64_t seed;
+    uint32_t limbs;
+    int32_t logN;
+};
+constexpr uint64_t FILL_CHUNK = 16384;   // words per workgroup
+// d_jobs: device array, word_begin ascending, starting at 0.
+int launch_fill_jobs(const FillJob* d_jobs, uint32_t njobs, uint64_t total_words, hipStream_t stream);
+
 }  // namespace ntt

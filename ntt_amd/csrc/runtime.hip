@@ -33,7 +33,41 @@ __global__ void fill_uniform_kernel(uint32_t* dst, const uint32_t* moduli, uint3
     }
 }
 
+// Many fills in one launch: workgroup b fills words [b * FILL_CHUNK, ...) of
+// the concatenation of all jobs (job j starts at word_begin[j]); each word
+// gets exactly the value fill_uniform_kernel gives it within its own job.
+__global__ void fill_uniform_jobs_kernel(const ntt::FillJob* jobs, uint32_t njobs, uint64_t total) {
+    const uint64_t begin = (uint64_t)blockIdx.x * ntt::FILL_CHUNK;
+    if (begin >= total) return;
+    const uint64_t end = begin + ntt::FILL_CHUNK < total ? begin + ntt::FILL_CHUNK : total;
+    uint32_t lo = 0, hi = njobs;   // last job with word_begin <= begin
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (jobs[mid].word_begin <= begin) lo = mid; else hi = mid;
+    }
+    for (uint64_t w = begin + threadIdx.x; w < end; w += blockDim.x) {
+        uint32_t j = lo;
+        while (j + 1 < njobs && jobs[j + 1].word_begin <= w) ++j;
+        const ntt::FillJob& f = jobs[j];
+        const uint64_t i = w - f.word_begin;
+        const uint32_t limb = (uint32_t)((i >> f.logN) % f.limbs);
+        const uint64_t h = splitmix64(f.seed ^ splitmix64(i));
+        f.dst[i] = (uint32_t)__umul64hi(h, (uint64_t)f.moduli[limb]);
+    }
+}
+
 }  // namespace
+
+namespace ntt {
+int launch_fill_jobs(const FillJob* d_jobs, uint32_t njobs, uint64_t total_words, hipStream_t stream) {
+    if (!njobs || !total_words) return NTT_OK;
+    const uint64_t blocks = (total_words + FILL_CHUNK - 1) / FILL_CHUNK;
+    if (blocks > 0x7fffffffull) return NTT_ERR_INVALID_ARGUMENT;
+    hipLaunchKernelGGL(fill_uniform_jobs_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, d_jobs, njobs,
+                       total_words);
+    return ck(hipGetLastError(), "fill_uniform_jobs launch");
+}
+}  // namespace ntt
 
 extern "C" {
 

@@ -122,6 +122,8 @@ struct Slot {
     char* h_limb = nullptr;
     char* d_limb = nullptr;
     size_t limb_cap = 0;            // limbs
+    FillJob* h_fill = nullptr;      // seeded-operand fills of a launch, pinned staging
+    FillJob* d_fill = nullptr;
     bool busy = false;
     std::vector<std::shared_ptr<Req>> reqs;
 };
@@ -163,6 +165,8 @@ struct ntt_server {
             if (s.d_desc) (void)hipFree(s.d_desc);
             if (s.h_limb) (void)hipHostFree(s.h_limb);
             if (s.d_limb) (void)hipFree(s.d_limb);
+            if (s.h_fill) (void)hipHostFree(s.h_fill);
+            if (s.d_fill) (void)hipFree(s.d_fill);
         }
         if (arena) (void)hipFree(arena);
     }
@@ -208,7 +212,9 @@ int plan_for(ntt_server* S, const ntt_request& r, ntt_plan** out) {
 int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req>>& reqs, uint64_t* fused,
                    uint64_t* two_pass) {
     int rc = NTT_OK;
-    // 1. operands into the arena
+    // 1. operands into the arena: copies, and every seeded fill in one launch
+    uint32_t nfill = 0;
+    uint64_t fill_words = 0;
     for (const auto& q : reqs) {
         uint32_t* dst = S->arena + q->off / sizeof(uint32_t);
         const ntt_request& r = q->r;
@@ -216,7 +222,16 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
                                 "hipMemcpyAsync(src_dev)");
         else if (r.src_host) rc = ck(hipMemcpyAsync(dst, r.src_host, q->bytes, hipMemcpyHostToDevice, s.stream),
                                      "hipMemcpyAsync(src_host)");
-        else rc = ntt_rt_fill_uniform(dst, q->plan->d_moduli, r.limbs, r.logN, 1, r.seed, s.stream);
+        else {
+            s.h_fill[nfill++] = FillJob{dst, q->plan->d_moduli, fill_words, r.seed, r.limbs, r.logN};
+            fill_words += (uint64_t)r.limbs << r.logN;
+        }
+        if (rc) return rc;
+    }
+    if (nfill) {
+        rc = ck(hipMemcpyAsync(s.d_fill, s.h_fill, nfill * sizeof(FillJob), hipMemcpyHostToDevice, s.stream),
+                "hipMemcpyAsync(fills)");
+        if (rc == NTT_OK) rc = launch_fill_jobs(s.d_fill, nfill, fill_words, s.stream);
         if (rc) return rc;
     }
     // 2. group by (logN, direction)
@@ -471,6 +486,13 @@ int ntt_server_create(ntt_context* ctx, const ntt_server_config* cfg, ntt_server
         if ((rc = ck(hipHostMalloc((void**)&s.h_desc, db, hipHostMallocDefault), "hipHostMalloc"))) return rc;
         if (hipMalloc(&s.d_desc, db) != hipSuccess) {
             s.d_desc = nullptr;
+            return NTT_ERR_OUT_OF_MEMORY;
+        }
+        if ((rc = ck(hipHostMalloc((void**)&s.h_fill, cfg->max_jobs_per_launch * sizeof(FillJob), hipHostMallocDefault),
+                     "hipHostMalloc")))
+            return rc;
+        if (hipMalloc(&s.d_fill, cfg->max_jobs_per_launch * sizeof(FillJob)) != hipSuccess) {
+            s.d_fill = nullptr;
             return NTT_ERR_OUT_OF_MEMORY;
         }
         s.limb_cap = (size_t)cfg->max_jobs_per_launch * SLOT_LIMBS_PER_JOB;
