@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="use the default phase config instead of the autotuner's")
     ap.add_argument("--engine", choices=["auto", "fused", "two_pass"], default="auto",
                     help="force a kernel family for the batched leg")
+    ap.add_argument("--phase-config", default=None,
+                    help="fixed two-pass config 'R,S,W;R,S,W;R,S,W;R,S,W' (ntt1;ntt2;intt1;intt2), for profiling")
     ap.add_argument("--fused-shape", type=int, choices=[0, 6, 7], default=0,
                     help="force the fused workgroup shape (0: the tuned one)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -289,13 +291,17 @@ def main():
         tuning["single"] = ctx.autotune(logN, moduli, count=1, reps=5)
         tuning["seconds"] = round(time.perf_counter() - t0, 2)
         log(f"autotune: {json.dumps(tuning)}")
-    plan = ctx.plan(logN, moduli, ntt_amd.MONTGOMERY)
+    fixed_cfg = None
+    if args.phase_config:
+        fixed_cfg = ntt_amd.TransformConfig.make(*[tuple(int(v) for v in ph.split(",")) for ph in
+                                                   args.phase_config.split(";")])
+    plan = ctx.plan(logN, moduli, ntt_amd.MONTGOMERY, fixed_cfg)
     if args.engine != "auto":
         plan.set_engine({"fused": ntt_amd.ENGINE_FUSED, "two_pass": ntt_amd.ENGINE_TWO_PASS}[args.engine])
     if args.fused_shape:
         plan.set_fused_shape(args.fused_shape)
     engine_used = {1: "two_pass", 2: "fused"}.get(plan.engine(B), "?")
-    cfg = ntt_amd.TransformConfig.make(*tuning["batch"]["cfg"]) if tuning else plan.config
+    cfg = plan.config if fixed_cfg or not tuning else ntt_amd.TransformConfig.make(*tuning["batch"]["cfg"])
     words_per_op = limbs * N
     pool = dev.DeviceBuffer(4 * words_per_op * B)
     qdev = dev.DeviceBuffer.from_numpy(np.array([m[0] for m in moduli], np.uint32))

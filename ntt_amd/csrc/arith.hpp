@@ -11,6 +11,15 @@
 
 namespace ntt {
 
+// Global-memory pointers: accesses through them use an SGPR base plus a
+// 32-bit VGPR offset (global_load/global_store) instead of a 64-bit flat
+// address per lane.
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef __attribute__((address_space(1))) const uint32_t gcu32;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4 gu4;
+typedef __attribute__((address_space(1))) const u32x4 gcu4;
+
 // (a + b) mod q for a, b in [0, q): a + b < 2q < 2^32; if a + b < q the
 // subtraction wraps to a value above 2^31 and min keeps a + b.
 __device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b, uint32_t q) {

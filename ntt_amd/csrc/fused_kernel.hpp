@@ -43,13 +43,6 @@ namespace fused {
 
 constexpr uint32_t LDS_WORDS = 32768 + 1024;
 
-// Global-memory pointers: loads through them use an SGPR base plus a 32-bit
-// VGPR offset instead of a 64-bit flat address per lane.
-typedef __attribute__((address_space(1))) uint32_t gu32;
-typedef __attribute__((address_space(1))) const uint32_t gcu32;
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-typedef __attribute__((address_space(1))) u32x4 gu4;
-typedef __attribute__((address_space(1))) const u32x4 gcu4;
 
 __host__ __device__ constexpr int nrounds(int n, int eb) { return n <= 2 * eb - 1 ? 2 : 3; }
 

@@ -80,7 +80,7 @@ __device__ __forceinline__ void wave_lds_sync() {
 
 // Stages on window bits [jlo, jhi] (forward: from jhi down; inverse: from jlo up).
 template <int S, bool INV, int MODE>
-__device__ __forceinline__ void round_stages(uint32_t (&v)[1 << S], const uint32_t* __restrict__ tw,
+__device__ __forceinline__ void round_stages(uint32_t (&v)[1 << S], gcu32* __restrict__ tw,
                                              uint32_t TB, int jlo, int jhi, uint32_t q,
                                              uint32_t q_inv, uint32_t r2) {
     constexpr int E = 1 << S;
@@ -129,8 +129,8 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     const uint32_t op = rest % p.count;
     const uint32_t li = rest / p.count;
     const size_t N = (size_t)1 << n;
-    uint32_t* A = p.A_ptrs ? p.A_ptrs[li] : p.A + ((size_t)op * p.limbs + li) * N;
-    const uint32_t* tw = p.tw_ptrs ? p.tw_ptrs[li] : p.tw + (size_t)li * N;
+    gu32* A = (gu32*)(p.A_ptrs ? p.A_ptrs[li] : p.A + ((size_t)op * p.limbs + li) * N);
+    gcu32* tw = (gcu32*)(p.tw_ptrs ? p.tw_ptrs[li] : p.tw + (size_t)li * N);
 
     const uint32_t q = p.moduli[li];
     const uint32_t q_inv = p.q_inv ? p.q_inv[li] : neg_inv_mod_2_32(q);
@@ -185,10 +185,10 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     // means every thread owns 2^S consecutive words (16-byte vector access).
     auto gload = [&](int lo, uint32_t (&v)[E]) {
         if (!COL && lo == 0 && E >= 4) {
-            const uint4* src = reinterpret_cast<const uint4*>(A + gaddr(vidx(0, 0)));
+            gcu4* src = (gcu4*)(A + gaddr(vidx(0, 0)));
 #pragma unroll
             for (int x = 0; x < E / 4; ++x) {
-                const uint4 w4 = src[x];
+                const u32x4 w4 = src[x];
                 v[4 * x] = w4.x; v[4 * x + 1] = w4.y; v[4 * x + 2] = w4.z; v[4 * x + 3] = w4.w;
             }
         } else {
@@ -198,10 +198,10 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     };
     auto gstore = [&](int lo, const uint32_t (&v)[E]) {
         if (!COL && lo == 0 && E >= 4) {
-            uint4* dst = reinterpret_cast<uint4*>(A + gaddr(vidx(0, 0)));
+            gu4* dst = (gu4*)(A + gaddr(vidx(0, 0)));
 #pragma unroll
             for (int x = 0; x < E / 4; ++x)
-                dst[x] = make_uint4(v[4 * x], v[4 * x + 1], v[4 * x + 2], v[4 * x + 3]);
+                dst[x] = u32x4{v[4 * x], v[4 * x + 1], v[4 * x + 2], v[4 * x + 3]};
         } else {
 #pragma unroll
             for (int x = 0; x < E; ++x) A[gaddr(vidx(lo, x))] = v[x];
