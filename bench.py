@@ -359,10 +359,14 @@ def main():
     else:
         mode = 2 if lazy else 0
         c = cfg.astuple()
-        kern = {f"phase_kernel<{c[0][1]}, false, true, {mode}>": lambda: plan.run_pass(pool, B, 0, 0, stream),
-                f"phase_kernel<{c[1][1]}, false, false, {mode}>": lambda: plan.run_pass(pool, B, 0, 1, stream),
-                f"phase_kernel<{c[2][1]}, true, false, {mode}>": lambda: plan.run_pass(pool, B, 1, 0, stream),
-                f"phase_kernel<{c[3][1]}, true, true, {mode}>": lambda: plan.run_pass(pool, B, 1, 1, stream)}
+        def pk(ph, inv, col):   # kernel symbol: the radix is a template argument for row passes, R 8-10, S >= 4
+            r, sm = c[ph][0], c[ph][1]
+            rf = r if (sm >= 4 and r in (8, 9, 10) and col == "false") else 0
+            return f"phase_kernel<{sm}, {inv}, {col}, {mode}, {rf}>"
+        kern = {pk(0, "false", "true"): lambda: plan.run_pass(pool, B, 0, 0, stream),
+                pk(1, "false", "false"): lambda: plan.run_pass(pool, B, 0, 1, stream),
+                pk(2, "true", "false"): lambda: plan.run_pass(pool, B, 1, 0, stream),
+                pk(3, "true", "true"): lambda: plan.run_pass(pool, B, 1, 1, stream)}
     names = list(kern)
     ev = [dev.Event() for _ in range(len(names) + 1)]
     per = {n: [] for n in names}

@@ -19,30 +19,43 @@ namespace ntt {
 
 using KernelFn = void (*)(PhaseParams);
 
+template <int S, int MODE, int RF>
+static KernelFn pick_r(bool inv, bool col) {
+    if (!inv && col) return phase_kernel<S, false, true, MODE, RF>;
+    if (!inv && !col) return phase_kernel<S, false, false, MODE, RF>;
+    if (inv && col) return phase_kernel<S, true, true, MODE, RF>;
+    return phase_kernel<S, true, false, MODE, RF>;
+}
+
 template <int S, int MODE>
-static KernelFn pick_m(bool inv, bool col) {
-    if (!inv && col) return phase_kernel<S, false, true, MODE>;
-    if (!inv && !col) return phase_kernel<S, false, false, MODE>;
-    if (inv && col) return phase_kernel<S, true, true, MODE>;
-    return phase_kernel<S, true, false, MODE>;
+static KernelFn pick_m(bool inv, bool col, int R) {
+    // Compile-time radix for the row-pass splits the tuner picks at logN
+    // 16-17 (measured 8-15 % faster there; the column passes measured no
+    // better or slower, so they keep the runtime radix).
+    if (MODE != MODE_TWCONV && S >= 4 && !col) {
+        if (R == 8) return pick_r<S, MODE, 8>(inv, col);
+        if (R == 9) return pick_r<S, MODE, 9>(inv, col);
+        if (R == 10) return pick_r<S, MODE, 10>(inv, col);
+    }
+    return pick_r<S, MODE, 0>(inv, col);
 }
 
 template <int S>
-static KernelFn pick_s(bool inv, bool col, int mode) {
+static KernelFn pick_s(bool inv, bool col, int mode, int R) {
     switch (mode) {
-        case MODE_CANON: return pick_m<S, MODE_CANON>(inv, col);
-        case MODE_TWCONV: return pick_m<S, MODE_TWCONV>(inv, col);
-        default: return pick_m<S, MODE_LAZY>(inv, col);
+        case MODE_CANON: return pick_m<S, MODE_CANON>(inv, col, R);
+        case MODE_TWCONV: return pick_m<S, MODE_TWCONV>(inv, col, R);
+        default: return pick_m<S, MODE_LAZY>(inv, col, R);
     }
 }
 
-static KernelFn pick(int S, bool inv, bool col, int mode) {
+static KernelFn pick(int S, bool inv, bool col, int mode, int R) {
     switch (S) {
-        case 1: return pick_s<1>(inv, col, mode);
-        case 2: return pick_s<2>(inv, col, mode);
-        case 3: return pick_s<3>(inv, col, mode);
-        case 4: return pick_s<4>(inv, col, mode);
-        case 5: return pick_s<5>(inv, col, mode);
+        case 1: return pick_s<1>(inv, col, mode, R);
+        case 2: return pick_s<2>(inv, col, mode, R);
+        case 3: return pick_s<3>(inv, col, mode, R);
+        case 4: return pick_s<4>(inv, col, mode, R);
+        case 5: return pick_s<5>(inv, col, mode, R);
         default: return nullptr;
     }
 }
@@ -81,7 +94,7 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, int mode,
     p.xcd_remap = (base.xcd_remap && blocks % 8 == 0) ? 1 : 0;
     const unsigned threads = 1u << (R - S + W);
     const size_t lds = lds_bytes_for((size_t)1 << (R + W));
-    KernelFn fn = pick(S, inv, col, mode);
+    KernelFn fn = pick(S, inv, col, mode, R);
     if (!fn) return NTT_ERR_CONFIG;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),

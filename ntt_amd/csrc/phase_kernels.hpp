@@ -110,12 +110,14 @@ __device__ __forceinline__ void round_stages(uint32_t (&v)[1 << S], gcu32* __res
 
 // One pass.  COL: column pass (outer stages; forward phase 1 / inverse phase 2).
 // !COL: row pass (inner stages; forward phase 2 / inverse phase 1).
-template <int S, bool INV, bool COL, int MODE>
+// RF > 0: the pass's radix stages fixed at compile time (RF == p.R), so the
+// round windows and every index below fold to constants and shifts.
+template <int S, bool INV, bool COL, int MODE, int RF = 0>
 __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     constexpr int E = 1 << S;
     extern __shared__ uint32_t lds[];
 
-    const int n = p.logN, R = p.R, W = p.W;
+    const int n = p.logN, R = RF ? RF : p.R, W = p.W;
     const uint32_t t = threadIdx.x;
 
     // ---- workgroup -> (limb index, operand, tile) ------------------------
@@ -212,6 +214,7 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     uint32_t v[E];
     const int K = (R + S - 1) / S;          // rounds
     const int rem = R - S * (K - 1);        // stages in the last round (1..S)
+#pragma unroll
 
     // Forward: full rounds from the top window down, the partial round last
     // (window [0, S), stages on window bits rem-1 .. 0).  Inverse: full rounds
