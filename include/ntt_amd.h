@@ -50,9 +50,9 @@ typedef enum { NTT_FORWARD = 0, NTT_INVERSE = 1 } ntt_direction;
 
 /* Kernel family a plan launches on.  TWO_PASS: the reference's two-phase
  * structure (column pass + row pass, one HBM round trip each).  FUSED: one
- * 1024-thread workgroup per 2^16 words holds whole limbs in registers and LDS,
- * one HBM round trip per transform (10 <= logN <= 16).  AUTO: the autotuned
- * choice for the launch size, else FUSED where supported. */
+ * persistent workgroup per CU holds 2^16 words (one or more whole limbs) in
+ * registers and LDS, one HBM round trip per transform (10 <= logN <= 16).
+ * AUTO: the autotuned choice for the launch size, else FUSED where supported. */
 typedef enum { NTT_ENGINE_AUTO = 0, NTT_ENGINE_TWO_PASS = 1, NTT_ENGINE_FUSED = 2 } ntt_engine;
 
 /* ModulusConfig, src/ntt_config.h:6-9 */
@@ -243,7 +243,7 @@ typedef struct {
     int candidates;             /* configurations timed */
     int engine;                 /* ntt_engine chosen (TWO_PASS or FUSED) */
     double fused_fwd_ms, fused_inv_ms;       /* best fused engine times (0 if not applicable) */
-    int fused_stagger;          /* start stagger chosen for the fused engine */
+    int fused_stagger;          /* start stagger of the fused engine (not searched: 0) */
     int fused_eb;               /* fused forward shape: 7 = 512 threads x 128 words, 6 = 1024 x 64 */
     int fused_eb_inv;           /* fused inverse shape */
     int fused_lazy;             /* fused forward arithmetic: 1 lazy, 0 canonical */
