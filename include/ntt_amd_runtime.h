@@ -64,6 +64,15 @@ int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream);
  * writes each byte once; the reference kernel for bandwidth and for scaling
  * the HBM byte counters. */
 int ntt_rt_stream_copy(void* dst, const void* src, size_t bytes, void* stream);
+/* hipGraph capture of any sequence of this library's asynchronous calls on
+ * `stream` (launchers, plan runs, batches, copies): begin, enqueue the calls,
+ * end to get a replayable graph, launch it any number of times, destroy it.
+ * The stream must not be the legacy default stream (it cannot be captured).
+ * Operand addresses are baked into the graph. */
+int ntt_rt_graph_begin(void* stream);
+int ntt_rt_graph_end(void* stream, void** graph);
+int ntt_rt_graph_launch(void* graph, void* stream);
+int ntt_rt_graph_destroy(void* graph);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,7 @@ __global__ void fill_uniform_jobs_kernel(const ntt::FillJob* jobs, uint32_t njob
 }  // namespace
 
 namespace ntt {
+int fused_prepare();
 int launch_fill_jobs(const FillJob* d_jobs, uint32_t njobs, uint64_t total_words, hipStream_t stream) {
     if (!njobs || !total_words) return NTT_OK;
     const uint64_t blocks = (total_words + FILL_CHUNK - 1) / FILL_CHUNK;
@@ -157,6 +158,50 @@ int ntt_rt_event_synchronize(void* event) {
 int ntt_rt_event_elapsed_ms(float* ms, void* start, void* stop) {
     if (!ms) return NTT_ERR_INVALID_ARGUMENT;
     return ck(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop), "hipEventElapsedTime");
+}
+
+namespace {
+struct CapturedGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+}  // namespace
+
+int ntt_rt_graph_begin(void* stream) {
+    if (!stream) return NTT_ERR_INVALID_ARGUMENT;
+    // kernel attributes are set once per process before any capture
+    int rc = ntt::fused_prepare();
+    if (rc) return rc;
+    return ck(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+}
+
+int ntt_rt_graph_end(void* stream, void** graph) {
+    if (!stream || !graph) return NTT_ERR_INVALID_ARGUMENT;
+    *graph = nullptr;
+    CapturedGraph* g = new CapturedGraph();
+    int rc = ck(hipStreamEndCapture((hipStream_t)stream, &g->graph), "hipStreamEndCapture");
+    if (rc == NTT_OK) rc = ck(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (rc) {
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return rc;
+    }
+    *graph = g;
+    return NTT_OK;
+}
+
+int ntt_rt_graph_launch(void* graph, void* stream) {
+    if (!graph) return NTT_ERR_INVALID_ARGUMENT;
+    return ck(hipGraphLaunch(static_cast<CapturedGraph*>(graph)->exec, (hipStream_t)stream), "hipGraphLaunch");
+}
+
+int ntt_rt_graph_destroy(void* graph) {
+    if (!graph) return NTT_ERR_INVALID_ARGUMENT;
+    CapturedGraph* g = static_cast<CapturedGraph*>(graph);
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return NTT_OK;
 }
 
 int ntt_rt_fill_uniform(uint32_t* dst, const uint32_t* moduli_dev, size_t limbs, int logN, size_t count,

@@ -44,6 +44,10 @@ _PROTOS = {
     "ntt_rt_fill_uniform": [_P, _P, _SZ, _I, _SZ, ctypes.c_uint64, _P],
     "ntt_rt_butterfly_roof": [_I, ctypes.POINTER(ctypes.c_double), _P],
     "ntt_rt_stream_copy": [_P, _P, _SZ, _P],
+    "ntt_rt_graph_begin": [_P],
+    "ntt_rt_graph_end": [_P, ctypes.POINTER(_P)],
+    "ntt_rt_graph_launch": [_P, _P],
+    "ntt_rt_graph_destroy": [_P],
 }
 for _n, _a in _PROTOS.items():
     _f = getattr(_lib, _n)
@@ -204,3 +208,48 @@ def stream_copy(dst, src, nbytes: int, stream=None):
     """Plain streaming copy (each byte read once, written once)."""
     _ck(_lib.ntt_rt_stream_copy(dst.ptr if hasattr(dst, "ptr") else dst, src.ptr if hasattr(src, "ptr") else src,
                                 nbytes, _sh(stream)), "stream_copy")
+
+
+class Graph:
+    """A captured sequence of library calls on a stream (hipGraph), replayable.
+
+        with Graph.capture(stream) as g:     # calls enqueued here are recorded
+            plan.forward(buf, 1, stream)
+        g.launch(stream)
+    """
+
+    def __init__(self):
+        self.handle = None
+        self._stream = None
+
+    @classmethod
+    def capture(cls, stream):
+        g = cls()
+        g._stream = stream
+        return g
+
+    def __enter__(self):
+        _ck(_lib.ntt_rt_graph_begin(_sh(self._stream)), "graph_begin")
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = ctypes.c_void_p()
+        rc = _lib.ntt_rt_graph_end(_sh(self._stream), ctypes.byref(h))
+        if exc_type is None:
+            _ck(rc, "graph_end")
+            self.handle = h.value
+        return False
+
+    def launch(self, stream=None):
+        _ck(_lib.ntt_rt_graph_launch(self.handle, _sh(stream)), "graph_launch")
+
+    def close(self):
+        if self.handle:
+            _lib.ntt_rt_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -357,3 +357,34 @@ def test_plan_constants_prologue_epilogue(env, logN, engine, domain):
     plan.inverse(A)
     gu.sync()
     assert np.array_equal(gu.to_host(A).reshape(x.shape), x)
+
+
+def test_graph_capture_of_plan_launches(env):
+    """A restore copy plus forward and inverse plan launches captured with
+    ntt_rt_graph_*: every replay (with eager work between replays) equals the
+    eager result."""
+    ntt_amd, gu, orc = env
+    logN, limbs = 14, 6
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    x = gu.random_operand(logN, moduli, 31)
+    src, A = gu.to_dev(x), gu.to_dev(x)
+    ctx = ntt_amd.Context(0)
+    plan = ctx.plan(logN, moduli, 1)
+    s = ntt_amd.Stream()
+    exp = orc.forward(x, logN, moduli, 1)
+    for engine in (ntt_amd.ENGINE_FUSED, ntt_amd.ENGINE_TWO_PASS):
+        plan.set_engine(engine)
+        with ntt_amd.device.Graph.capture(s) as g:
+            A.copy_from(src, stream=s)
+            plan.forward(A, 1, s)
+        _print("case graph", engine)
+        for r in range(3):
+            A.zero()
+            gu.sync()
+            g.launch(s)
+            s.synchronize()
+            assert np.array_equal(gu.to_host(A).reshape(x.shape), exp), (engine, r)
+            plan.inverse(A, 1, s)              # eager work between replays
+            s.synchronize()
+            assert np.array_equal(gu.to_host(A).reshape(x.shape), x)
+        g.close()
