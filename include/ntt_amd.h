@@ -336,7 +336,8 @@ typedef struct {
     const uint32_t* src_host;
     const uint32_t* src_dev;
     uint64_t seed;                  /* both sources NULL: uniform residues (ntt_rt_fill_uniform) */
-    int32_t auto_release;           /* 1: free the operand when done (result not kept) */
+    int32_t auto_release;           /* 1: free the operand when done (result not kept); the
+                                       request stays known to state/wait until released */
 } ntt_request;
 
 typedef enum {

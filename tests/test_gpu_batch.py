@@ -167,3 +167,24 @@ def test_batch_full_uneven_workload(env):
     s.synchronize()
     for k in range(len(wl)):
         assert np.array_equal(gu.to_host(bufs[k]), xs[k]), f"roundtrip job {k}"
+
+
+def test_batch_many_small_jobs(env):
+    """600 one-limb jobs of sizes 10..13 in both directions (a few fused
+    groups with hundreds of job descriptors each), oracle-checked."""
+    ntt_amd, gu, orc = env
+    from ntt_amd import workloads
+    rng = np.random.default_rng(21)
+    jobs = []
+    for k in range(600):
+        logN = int(rng.integers(10, 14))
+        m = workloads.moduli_for(logN, 1)
+        x = gu.random_operand(logN, m, 4000 + k)
+        jobs.append(dict(logN=logN, moduli=m, direction=int(rng.integers(0, 2)), domain=1, x=x, buf=gu.to_dev(x)))
+    ctx = ntt_amd.Context(0)
+    b = ctx.batch([(j["buf"], j["logN"], j["moduli"], j["direction"], j["domain"]) for j in jobs])
+    _print("case batch-many", b.info())
+    s = ntt_amd.Stream()
+    b.replay(s)
+    s.synchronize()
+    _check_all(env, jobs)

@@ -141,3 +141,40 @@ def test_server_poisson_trace(env):
     assert st["mean_jobs_per_launch"] > 1.0
     assert st["max_slots_busy"] >= 2
     srv.close()
+
+
+def test_server_fifo_admission_order(env):
+    """Budget for one operand at a time: requests are admitted and finished
+    strictly in arrival order, and every result still matches the oracle."""
+    ntt_amd, gu, orc = env
+    ctx = ntt_amd.Context(0)
+    logN, limbs = 13, 2
+    op = 4 * (limbs << logN)
+    srv = ctx.server(hbm_budget_bytes=op, slots=3, max_jobs_per_launch=4)
+    _print("case server-fifo")
+    ids = [srv.submit(logN, limbs, k % 2, seed=300 + k) for k in range(6)]
+    for k, rid in enumerate(ids):
+        srv.wait(rid)
+        # the next request cannot have been admitted while this result is held
+        if k + 1 < len(ids):
+            assert srv.state(ids[k + 1]) == ntt_amd.REQ_QUEUED
+        x, m = _seeded_input(ntt_amd, gu, logN, limbs, 300 + k)
+        assert np.array_equal(srv.fetch(rid, logN, limbs), _oracle(orc, x, logN, m, k % 2))
+    assert srv.stats()["arena_peak_bytes"] <= op
+    srv.close()
+
+
+def test_server_argument_errors(env):
+    ntt_amd, gu, orc = env
+    ctx = ntt_amd.Context(0)
+    with pytest.raises(ntt_amd.NTTError):
+        ctx.server(hbm_budget_bytes=1 << 20, slots=0)
+    srv = ctx.server(hbm_budget_bytes=1 << 20, slots=1, max_jobs_per_launch=2)
+    for bad in (dict(logN=0, limbs=1), dict(logN=12, limbs=0), dict(logN=25, limbs=1)):
+        with pytest.raises(ntt_amd.NTTError):
+            srv.submit(bad["logN"], bad["limbs"])
+    with pytest.raises(ntt_amd.NTTError):
+        srv.submit(12, 1, direction=2)
+    with pytest.raises(ntt_amd.NTTError):
+        srv.wait(12345)                       # unknown id
+    srv.close()
