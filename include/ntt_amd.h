@@ -218,6 +218,31 @@ int ntt_plan_inverse(ntt_plan* plan, uint32_t* A_dev, void* stream);
  * (operand k at A_dev + k * limbs * N) in one launch per phase. */
 int ntt_plan_forward_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream);
 int ntt_plan_inverse_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream);
+/* Out-of-place forms: read `count` operands at src_dev, write their
+ * transforms to dst_dev (same layout); src_dev is left unchanged unless it
+ * equals dst_dev (then the call is the in-place one).  The two ranges must be
+ * the same or disjoint.  The src -> dst form of Cheddar's NTTForModUp
+ * (NTT_cheddar.cu:815-888: NTTPhase1 reads `src`, writes `dst`) and
+ * INTTForModDown (974-1128: dst = INTT(src) * const, with the plan's epilogue
+ * constants of ntt_plan_set_constants).  A limb range that ModUp skips is a
+ * plan over the remaining moduli, called on the matching limb offsets. */
+int ntt_plan_forward_to(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, size_t count,
+                        void* stream);
+int ntt_plan_inverse_to(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, size_t count,
+                        void* stream);
+/* Per-limb factors f_l (host array of `limbs` words, reduced mod q_l) of
+ * ntt_plan_forward_mod_down, e.g. P^-1 mod q_l; NULL releases them.
+ * Synchronous; the first call allocates `limbs` words of device memory. */
+int ntt_plan_set_mod_down_factors(ntt_plan* plan, const uint32_t* factors);
+/* dst = (sub - NTT(src)) * f_l, limb by limb, in the forward transform's
+ * output order and domain (sub holds canonical residues in that order and
+ * domain): the NTT and the subtract-and-scale of Cheddar's NTTForModDown
+ * (NTT_cheddar.cu:889-972; the epilogue of NTTPhase2ForModDown, 573-599) in
+ * one pass over the data.  src may equal dst (in place); sub must be disjoint
+ * from dst (the reference asserts dst != src2, NTT_cheddar.cu:910-911).
+ * NTT_ERR_INVALID_ARGUMENT until ntt_plan_set_mod_down_factors was called. */
+int ntt_plan_forward_mod_down(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev,
+                              const uint32_t* sub_dev, size_t count, void* stream);
 /* One of the two passes alone (pass 0 runs first: the column pass of a
  * forward transform, the row pass of an inverse one); the two in order equal
  * the whole transform.  For per-kernel timing, and for callers that put work

@@ -205,6 +205,10 @@ _PROTOS = {
     "ntt_plan_inverse": (_I, [_P, _P, _P]),
     "ntt_plan_forward_many": (_I, [_P, _P, _SZ, _P]),
     "ntt_plan_inverse_many": (_I, [_P, _P, _SZ, _P]),
+    "ntt_plan_forward_to": (_I, [_P, _P, _P, _SZ, _P]),
+    "ntt_plan_inverse_to": (_I, [_P, _P, _P, _SZ, _P]),
+    "ntt_plan_set_mod_down_factors": (_I, [_P, _PU32]),
+    "ntt_plan_forward_mod_down": (_I, [_P, _P, _P, _P, _SZ, _P]),
     "ntt_plan_run_pass": (_I, [_P, _P, _SZ, _I, _I, _P]),
     "ntt_plan_set_engine": (_I, [_P, _I]),
     "ntt_plan_set_fused_shape": (_I, [_P, _I]),
@@ -635,6 +639,30 @@ class Plan:
 
     def inverse(self, A, count: int = 1, stream=None):
         _check(_lib.ntt_plan_inverse_many(self.handle, _addr(A), count, _stream(stream)), "ntt_plan_inverse")
+
+    def forward_to(self, dst, src, count: int = 1, stream=None):
+        """dst = NTT(src) for `count` operands; src is unchanged unless it is dst."""
+        _check(_lib.ntt_plan_forward_to(self.handle, _addr(dst), _addr(src), count, _stream(stream)),
+               "ntt_plan_forward_to")
+
+    def inverse_to(self, dst, src, count: int = 1, stream=None):
+        """dst = INTT(src) (times the epilogue constants, if set)."""
+        _check(_lib.ntt_plan_inverse_to(self.handle, _addr(dst), _addr(src), count, _stream(stream)),
+               "ntt_plan_inverse_to")
+
+    def set_mod_down_factors(self, factors=None):
+        """Per-limb factors f of forward_mod_down (None releases them)."""
+        if factors is None:
+            _check(_lib.ntt_plan_set_mod_down_factors(self.handle, None), "ntt_plan_set_mod_down_factors")
+            return
+        a = (ctypes.c_uint32 * self.limbs)(*[int(x) for x in factors])
+        _check(_lib.ntt_plan_set_mod_down_factors(self.handle, ctypes.cast(a, _PU32)),
+               "ntt_plan_set_mod_down_factors")
+
+    def forward_mod_down(self, dst, src, sub, count: int = 1, stream=None):
+        """dst = (sub - NTT(src)) * f per limb, in the NTT output's order and domain."""
+        _check(_lib.ntt_plan_forward_mod_down(self.handle, _addr(dst), _addr(src), _addr(sub), count,
+                                              _stream(stream)), "ntt_plan_forward_mod_down")
 
     def set_engine(self, engine: int):
         _check(_lib.ntt_plan_set_engine(self.handle, engine), "ntt_plan_set_engine")

@@ -238,6 +238,7 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
             const ntt_plan* P = b->job_plan[i];
             FusedJob fj{};
             fj.data = j.data;
+            fj.src = j.data;
             fj.tw = g.inverse ? P->d_tw_inv : P->d_tw_fwd;
             fj.lc = g.inverse ? P->d_lc_inv : P->d_lc_fwd;
             fj.limbs = j.limbs;
@@ -388,6 +389,7 @@ int ntt_batch_set_data(ntt_batch* batch, size_t index, uint32_t* data) {
     }
     const size_t d = batch->job_desc[index];
     batch->desc[d].data = data;
+    batch->desc[d].src = data;
     return ck(hipMemcpy(batch->d_desc + d, &batch->desc[d], sizeof(FusedJob), hipMemcpyHostToDevice),
               "hipMemcpy(job)");
 }

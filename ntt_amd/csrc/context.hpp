@@ -74,6 +74,7 @@ struct ntt_plan {
     std::vector<const uint32_t*> h_tw_fwd, h_tw_inv;   // host copies of the table pointers
     std::vector<uint32_t> h_scale_fwd, h_scale_inv;    // host copies of the scale words
     bool prologue = false;                         // forward input multiplied by per-limb constants
+    uint32_t* d_post = nullptr;                    // mod-down multipliers to_mont(f_l), own allocation
 };
 
 namespace ntt {
@@ -87,7 +88,7 @@ int log2_bucket(size_t v);
 // Engine a launch of `count` operands runs on (resolves NTT_ENGINE_AUTO).
 int plan_engine(const ntt_plan* P, size_t count);
 int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream, int eb,
-                      bool lazy, int stagger);
+                      bool lazy, int stagger, const uint32_t* src = nullptr, const uint32_t* sub = nullptr);
 // The fused kernel variant a run of `count` operands uses in `inverse` direction.
 void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, int* eb, bool* lazy, int* stagger);
 // Fused variant for a launch of `total_limbs` limbs of size 2^logN from the

@@ -35,6 +35,10 @@ uint32_t fused_items(int logN, size_t limbs) {
 
 FusedFn fused_fn_eb6(int n, bool inv, bool lazy);   // fused_eb6.hip
 FusedFn fused_fn_eb7(int n, bool inv, bool lazy);   // fused_eb7.hip
+FusedFn fused_fn_md(int n, bool lazy);               // fused_md.hip
+
+// Window width of the mod-down kernels: 7 with lazy arithmetic, 6 canonical.
+static int md_eb(bool lazy) { return lazy ? 7 : 6; }
 
 static FusedFn fused_fn(int n, bool inv, int eb, bool lazy) {
     return eb == 6 ? fused_fn_eb6(n, inv, lazy) : fused_fn_eb7(n, inv, lazy);
@@ -54,18 +58,19 @@ int fused_prepare() {
                     attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
             }
+        for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
+            for (int lz = 0; lz < 2 && attr_err == hipSuccess; ++lz) {
+                FusedFn f = fused_fn_md(m, lz == 1);
+                if (f)
+                    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            }
     });
     return attr_err == hipSuccess ? NTT_OK : record_hip_error(attr_err, "hipFuncSetAttribute(fused)");
 }
 
-// Launches the persistent kernel of size 2^n / direction `inv`; every job in
-// p has that size and direction.
-int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream,
-                 unsigned grid_limit) {
-    if (p.total_items == 0) return NTT_OK;
-    if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
-    FusedFn fn = fused_fn(n, inv, eb, lazy);
-    if (!fn) return NTT_ERR_CONFIG;
+// One launch of fused kernel `fn` (window width eb) over the items of p.
+static int launch_fused_fn(FusedFn fn, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit) {
     int rc = fused_prepare();
     if (rc) return rc;
     const size_t lds = FUSED_LDS_WORDS * sizeof(uint32_t);
@@ -79,15 +84,33 @@ int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipSt
     return e == hipSuccess ? NTT_OK : record_hip_error(e, "fused kernel launch");
 }
 
-// `count` operands of a plan as one job (limb l uses table l % limbs).
+// Launches the persistent kernel of size 2^n / direction `inv`; every job in
+// p has that size and direction.
+int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream,
+                 unsigned grid_limit) {
+    if (p.total_items == 0) return NTT_OK;
+    if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
+    FusedFn fn = fused_fn(n, inv, eb, lazy);
+    if (!fn) return NTT_ERR_CONFIG;
+    return launch_fused_fn(fn, eb, p, stream, grid_limit);
+}
+
+// `count` operands of a plan as one job (limb l uses table l % limbs), read
+// from `src` (null: in place) and written to A; with `sub` non-null, the
+// forward mod-down form (the plan's per-limb multipliers).
 int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream, int eb,
-                      bool lazy, int stagger) {
+                      bool lazy, int stagger, const uint32_t* src, const uint32_t* sub) {
     if (!fused_supported(P->logN)) return NTT_ERR_CONFIG;
-    if (reinterpret_cast<uintptr_t>(A) & 15) return NTT_ERR_INVALID_ARGUMENT;
+    if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(sub)) & 15)
+        return NTT_ERR_INVALID_ARGUMENT;
+    if (sub && (inverse || !P->d_post)) return NTT_ERR_INVALID_ARGUMENT;
     const size_t limbs = P->limbs * count;
     if (limbs > 0xffffffffu) return NTT_ERR_INVALID_ARGUMENT;
     FusedParams p{};
     p.job0.data = A;
+    p.job0.src = src ? src : A;
+    p.job0.sub = sub;
+    p.job0.post = sub ? P->d_post : nullptr;
     p.job0.tw = inverse ? P->d_tw_inv : P->d_tw_fwd;
     p.job0.lc = inverse ? P->d_lc_inv : P->d_lc_fwd;
     p.job0.limbs = (uint32_t)limbs;
@@ -100,6 +123,13 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.total_items = fused_items(P->logN, limbs);
     p.counter = nullptr;
     p.stagger = stagger < 0 ? 0u : (uint32_t)stagger;
+    if (sub) {
+        const bool lz = P->lazy;   // the mod-down shapes: lazy when every q < 2^30
+        FusedFn fn = fused_fn_md(P->logN, lz);
+        if (!fn) return NTT_ERR_CONFIG;
+        if (p.total_items == 0) return NTT_OK;
+        return launch_fused_fn(fn, md_eb(lz), p, stream, 0);
+    }
     // lazy reduction needs every q < 2^30 and exists for the forward kernels
     return launch_fused(P->logN, inverse, eb, lazy && P->lazy && !inverse, p, stream, 0);
 }

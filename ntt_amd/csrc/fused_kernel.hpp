@@ -198,7 +198,7 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[1 << EB], uint32_t* L, ui
     });
 }
 
-template <int n, bool INV, bool LAZY, int EB>
+template <int n, bool INV, bool LAZY, int EB, bool MD>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds) {
     constexpr int E = 1 << EB;
     constexpr int G_BITS = n - EB;
@@ -216,7 +216,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     const bool valid = limb < job.limbs;
     const uint32_t lsafe = valid ? limb : first_limb;
     const uint32_t per = lsafe % job.period;
-    gu32* A = (gu32*)(job.data + ((size_t)lsafe << n));
+    gcu32* In = (gcu32*)(job.src + ((size_t)lsafe << n));
     gcu32* tw = (gcu32*)job.tw[per];
     const LimbConst c = job.lc[per];
     uint32_t* L = lds + slot * STRIDE;
@@ -225,7 +225,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     // ---- load (round 0 layout) ----
     if (valid) {
         if (LO0 == 0) {
-            gcu4* src = (gcu4*)(A + ubase<LO0, EB>(opaque(u)));
+            gcu4* src = (gcu4*)(In + ubase<LO0, EB>(opaque(u)));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
                 constexpr int x4 = decltype(X4)::value;
                 const u32x4 w = src[x4];
@@ -235,7 +235,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
                 v[REG<n, INV, 0, 4 * x4 + 3, EB>] = w.w;
             });
         } else {
-            gcu32* Ab = A + ubase<LO0, EB>(opaque(u));
+            gcu32* Ab = In + ubase<LO0, EB>(opaque(u));
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
                 v[REG<n, INV, 0, x, EB>] = Ab[x << LO0];
@@ -268,6 +268,29 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 #pragma unroll
             for (int x = 0; x < E; ++x) v[x] = canon_4q(v[x], c.q);
         }
+        if constexpr (MD) {
+            // mod-down: (sub - NTT) * post, read at the positions stored below
+            const uint32_t f = job.post[per];
+            gcu32* Sb = (gcu32*)(job.sub + ((size_t)lsafe << n)) + ubase<LOL, EB>(opaque(u));
+            if constexpr (LOL == 0) {
+                static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
+                    constexpr int x4 = decltype(X4)::value;
+                    const u32x4 s4 = ((gcu4*)Sb)[x4];
+                    const uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
+                    static_for<4>([&](auto K4) __attribute__((always_inline)) {
+                        constexpr int r = REG<n, INV, NR - 1, 4 * x4 + decltype(K4)::value, EB>;
+                        v[r] = mont_mul(sub_mod(sv[decltype(K4)::value], v[r], c.q), f, c.q, c.q_inv);
+                    });
+                });
+            } else {
+                static_for<E>([&](auto X) __attribute__((always_inline)) {
+                    constexpr int x = decltype(X)::value;
+                    constexpr int r = REG<n, INV, NR - 1, x, EB>;
+                    v[r] = mont_mul(sub_mod(Sb[x << LOL], v[r], c.q), f, c.q, c.q_inv);
+                });
+            }
+        }
+        gu32* A = (gu32*)(job.data + ((size_t)lsafe << n));
         if (LOL == 0) {
             gu4* dst = (gu4*)(A + ubase<LOL, EB>(opaque(u)));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
@@ -290,7 +313,8 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 // Persistent loop over work items of jobs of one size 2^n and one direction;
 // an item is 2^(16-n) consecutive limbs of one job.  Every thread of the
 // workgroup runs every barrier of every item.
-template <int n, bool INV, bool LAZY, int EB>
+// MD: forward mod-down form, storing (sub - NTT) * post (FusedJob).
+template <int n, bool INV, bool LAZY, int EB, bool MD = false>
 __global__ __launch_bounds__(1 << (16 - EB), 1) void fused_kernel(FusedParams p) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_next;
@@ -319,7 +343,7 @@ __global__ __launch_bounds__(1 << (16 - EB), 1) void fused_kernel(FusedParams p)
             }
             job = p.jobs[lo];
         }
-        fused::run_item<n, INV, LAZY, EB>(job, (item - job.item_begin) << (16 - n), lds);
+        fused::run_item<n, INV, LAZY, EB, MD>(job, (item - job.item_begin) << (16 - n), lds);
     }
 }
 

@@ -25,9 +25,15 @@ struct LimbConst {
 };
 
 // A run of `limbs` limbs of size 2^n stored back to back at `data`; limb l
-// uses table tw[l % period] and constants lc[l % period].
+// uses table tw[l % period] and constants lc[l % period].  The input is read
+// from `src` (same layout; equal to `data` in place).  The mod-down kernels
+// also read `sub` (same layout as `data`) and the per-limb Montgomery
+// multipliers post[l % period], and store (sub - transform) * post.
 struct FusedJob {
     uint32_t* data;
+    const uint32_t* src;
+    const uint32_t* sub;
+    const uint32_t* post;
     const uint32_t* const* tw;
     const LimbConst* lc;
     uint32_t limbs;

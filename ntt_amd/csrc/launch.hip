@@ -114,6 +114,8 @@ int launch_transform(const TransformLaunch& L, hipStream_t stream) {
     for (size_t first = 0; first < L.count; first += L.chunk) {
         TransformLaunch c = L;
         c.A = L.A + first * words;
+        if (L.src) c.src = L.src + first * words;
+        if (L.sub) c.sub = L.sub + first * words;
         c.count = L.count - first < L.chunk ? L.count - first : L.chunk;
         int rc = launch_transform_passes(c, stream, 3);
         if (rc) return rc;
@@ -127,7 +129,12 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
     if (L.A_ptrs && (L.count != 1 || !L.tw_ptrs)) return NTT_ERR_INVALID_ARGUMENT;
     if (L.logN < 1 || L.logN > NTT_MAX_LOG_N) return NTT_ERR_INVALID_ARGUMENT;
     if (L.scale_mode != SCALE_NONE && !L.scale) return NTT_ERR_INVALID_ARGUMENT;
-    if (reinterpret_cast<uintptr_t>(L.A) & 15) return NTT_ERR_INVALID_ARGUMENT;   // 16 B vector access
+    if ((reinterpret_cast<uintptr_t>(L.A) | reinterpret_cast<uintptr_t>(L.src) | reinterpret_cast<uintptr_t>(L.sub)) &
+        15)
+        return NTT_ERR_INVALID_ARGUMENT;   // 16 B vector access
+    // out-of-place and mod-down launches run whole transforms on contiguous operands
+    if ((L.src || L.sub) && (L.A_ptrs || pass_mask != 3)) return NTT_ERR_INVALID_ARGUMENT;
+    if (L.sub && (L.inverse || !L.post)) return NTT_ERR_INVALID_ARGUMENT;
     int rc = validate_config(L.logN, &L.cfg, device_max_lds());
     if (rc) return rc;
 
@@ -159,6 +166,10 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
         PhaseParams a = first_is_col ? scaled : none, b = first_is_col ? none : scaled;
         a.final_pass = c2.radix_stages == 0;
         b.final_pass = 1;
+        (first_is_col ? a : b).src = L.src;
+        PhaseParams& fin = c2.radix_stages > 0 ? b : a;
+        fin.sub = L.sub;
+        fin.post = L.post;
         if ((pass_mask & 1) && (rc = launch_pass(c1, false, true, mode, a, stream))) return rc;
         if ((pass_mask & 2) && (rc = launch_pass(c2, false, false, mode, b, stream))) return rc;
     } else {
@@ -167,6 +178,7 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
         PhaseParams a = last_is_col ? none : scaled, b = last_is_col ? scaled : none;
         a.final_pass = c2.radix_stages == 0;
         b.final_pass = 1;
+        (c1.radix_stages > 0 ? a : b).src = L.src;
         if ((pass_mask & 1) && (rc = launch_pass(c1, true, false, mode, a, stream))) return rc;
         if ((pass_mask & 2) && (rc = launch_pass(c2, true, true, mode, b, stream))) return rc;
     }

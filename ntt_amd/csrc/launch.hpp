@@ -13,6 +13,9 @@ namespace ntt {
 struct TransformLaunch {
     uint32_t* A;               // operand(s), A[(op * limbs + l) * N + i] ...
     uint32_t* const* A_ptrs;   // ... or a device array of per-limb operand pointers (count == 1)
+    const uint32_t* src;       // input, same layout as A (null: A, in place); needs A
+    const uint32_t* sub;       // forward mod-down: A = (sub - NTT(src)) * post, same layout as A
+    const uint32_t* post;      // per-limb Montgomery multipliers of the mod-down form
     const uint32_t* tw;        // Montgomery-form (or, with twconv, normal-domain) table per limb
     const uint32_t* const* tw_ptrs;  // device array of per-limb table pointers (overrides tw)
     const uint32_t* moduli;    // q per limb

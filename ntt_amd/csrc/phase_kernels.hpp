@@ -45,6 +45,9 @@ enum ScaleMode : int {
 struct PhaseParams {
     uint32_t* A;              // operand base, operand k at A + k * limbs * N ...
     uint32_t* const* A_ptrs;  // ... or per-limb operand pointers (count == 1)
+    const uint32_t* src;      // this pass's input, same layout as A (null: A)
+    const uint32_t* sub;      // final forward pass, mod-down: store (sub - v) * post[l]
+    const uint32_t* post;
     const uint32_t* tw;       // twiddle table, limb l at tw + l * N ...
     const uint32_t* const* tw_ptrs;  // ... or per-limb table pointers (twiddle cache)
     const uint32_t* moduli;   // q per limb
@@ -131,7 +134,9 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     const uint32_t op = rest % p.count;
     const uint32_t li = rest / p.count;
     const size_t N = (size_t)1 << n;
-    gu32* A = (gu32*)(p.A_ptrs ? p.A_ptrs[li] : p.A + ((size_t)op * p.limbs + li) * N);
+    const size_t off = ((size_t)op * p.limbs + li) * N;
+    gu32* A = (gu32*)(p.A_ptrs ? p.A_ptrs[li] : p.A + off);
+    gcu32* In = p.src ? (gcu32*)(p.src + off) : A;
     gcu32* tw = (gcu32*)(p.tw_ptrs ? p.tw_ptrs[li] : p.tw + (size_t)li * N);
 
     const uint32_t q = p.moduli[li];
@@ -185,9 +190,9 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
 
     // Global tile I/O for a round with window `lo`; lo == 0 in a row pass
     // means every thread owns 2^S consecutive words (16-byte vector access).
-    auto gload = [&](int lo, uint32_t (&v)[E]) {
+    auto gload = [&](gcu32* G, int lo, uint32_t (&v)[E]) {
         if (!COL && lo == 0 && E >= 4) {
-            gcu4* src = (gcu4*)(A + gaddr(vidx(0, 0)));
+            gcu4* src = (gcu4*)(G + gaddr(vidx(0, 0)));
 #pragma unroll
             for (int x = 0; x < E / 4; ++x) {
                 const u32x4 w4 = src[x];
@@ -195,7 +200,7 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
             }
         } else {
 #pragma unroll
-            for (int x = 0; x < E; ++x) v[x] = A[gaddr(vidx(lo, x))];
+            for (int x = 0; x < E; ++x) v[x] = G[gaddr(vidx(lo, x))];
         }
     };
     auto gstore = [&](int lo, const uint32_t (&v)[E]) {
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
         const bool last = (k == K - 1);
         const int lo = INV ? (last ? R - S : S * k) : (last ? 0 : R - S * (k + 1));
         if (k == 0) {
-            gload(lo, v);
+            gload(In, lo, v);
             if (!INV && scaled) {
 #pragma unroll
                 for (int x = 0; x < E; ++x)
@@ -250,6 +255,14 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
                 // forward values lie in [0, 4q), unscaled inverse ones in [0, 2q)
 #pragma unroll
                 for (int x = 0; x < E; ++x) v[x] = canon_4q(v[x], q);
+            }
+            if (!INV && p.sub) {
+                // mod-down: (sub - NTT) * post, read at the positions stored below
+                const uint32_t f = p.post[li];
+                uint32_t s[E];
+                gload((gcu32*)(p.sub + off), lo, s);
+#pragma unroll
+                for (int x = 0; x < E; ++x) v[x] = mont_mul(sub_mod(s[x], v[x], q), f, q, q_inv);
             }
             gstore(lo, v);
         }

@@ -245,6 +245,7 @@ int ntt_plan_create(ntt_context* ctx, int logN, const ntt_modulus_config* moduli
 int ntt_plan_destroy(ntt_plan* plan) {
     if (!plan) return NTT_ERR_INVALID_ARGUMENT;
     if (plan->d_const) (void)hipFree(plan->d_const);
+    if (plan->d_post) (void)hipFree(plan->d_post);
     delete plan;
     return NTT_OK;
 }
@@ -255,15 +256,34 @@ int ntt_plan_get_config(const ntt_plan* plan, ntt_transform_config* out) {
     return NTT_OK;
 }
 
-static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse, void* stream) {
+// Two device ranges of `bytes` bytes are the same range or disjoint.
+static bool same_or_disjoint(const void* a, const void* b, size_t bytes) {
+    const uintptr_t x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
+    return x == y || x + bytes <= y || y + bytes <= x;
+}
+
+// `count` operands: in place (src null), out of place from src, or (forward,
+// sub non-null) the mod-down form A = (sub - NTT(src or A)) * f.
+static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse, void* stream,
+                    const uint32_t* src = nullptr, const uint32_t* sub = nullptr) {
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
+    const size_t bytes = (plan->limbs << plan->logN) * count * sizeof(uint32_t);
+    if (src == A_dev) src = nullptr;
+    if (src && !same_or_disjoint(src, A_dev, bytes)) return NTT_ERR_INVALID_ARGUMENT;
+    if (sub && (inverse || !plan->d_post)) return NTT_ERR_INVALID_ARGUMENT;
+    // sub is read in the last pass, after A already holds intermediate values
+    if (sub && (sub == A_dev || !same_or_disjoint(sub, A_dev, bytes))) return NTT_ERR_INVALID_ARGUMENT;
     if (plan_engine(plan, count) == NTT_ENGINE_FUSED) {
         int eb = 0, stagger = 0;
         bool lazy = false;
         plan_fused_variant(plan, count, inverse, &eb, &lazy, &stagger);
-        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, eb, lazy, stagger);
+        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, eb, lazy, stagger, src, sub);
     }
-    return launch_transform(plan_launch(plan, A_dev, count, inverse), (hipStream_t)stream);
+    TransformLaunch L = plan_launch(plan, A_dev, count, inverse);
+    L.src = src;
+    L.sub = sub;
+    L.post = sub ? plan->d_post : nullptr;
+    return launch_transform(L, (hipStream_t)stream);
 }
 
 int ntt_plan_forward_many(ntt_plan* plan, uint32_t* A_dev, size_t count, void* stream) {
@@ -346,6 +366,47 @@ int ntt_plan_run_pass(ntt_plan* plan, uint32_t* A_dev, size_t count, int directi
     if (direction != NTT_FORWARD && direction != NTT_INVERSE) return NTT_ERR_INVALID_ARGUMENT;
     return launch_transform_passes(plan_launch(plan, A_dev, count, direction == NTT_INVERSE), (hipStream_t)stream,
                                    1 << pass);
+}
+
+int ntt_plan_forward_to(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, size_t count, void* stream) {
+    if (!src_dev) return NTT_ERR_INVALID_ARGUMENT;
+    return plan_run(plan, dst_dev, count, false, stream, src_dev);
+}
+
+int ntt_plan_inverse_to(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, size_t count, void* stream) {
+    if (!src_dev) return NTT_ERR_INVALID_ARGUMENT;
+    return plan_run(plan, dst_dev, count, true, stream, src_dev);
+}
+
+int ntt_plan_set_mod_down_factors(ntt_plan* plan, const uint32_t* factors) {
+    if (!plan) return NTT_ERR_INVALID_ARGUMENT;
+    if (!factors) {
+        if (plan->d_post) (void)hipFree(plan->d_post);
+        plan->d_post = nullptr;
+        return NTT_OK;
+    }
+    const size_t L = plan->limbs;
+    std::vector<uint32_t> post(L);
+    for (size_t l = 0; l < L; ++l) {
+        // mont_mul(v, to_mont(f)) = v f mod q in either domain of v
+        const uint32_t q = plan->h_q[l];
+        post[l] = to_mont_host(factors[l] % q, q);
+    }
+    if (!plan->d_post) {
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&plan->d_post), L * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            plan->d_post = nullptr;
+            return record_hip_error(e, "hipMalloc(mod-down factors)");
+        }
+    }
+    hipError_t e = hipMemcpy(plan->d_post, post.data(), L * sizeof(uint32_t), hipMemcpyHostToDevice);
+    return e == hipSuccess ? NTT_OK : record_hip_error(e, "hipMemcpy(mod-down factors)");
+}
+
+int ntt_plan_forward_mod_down(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, const uint32_t* sub_dev,
+                              size_t count, void* stream) {
+    if (!src_dev || !sub_dev) return NTT_ERR_INVALID_ARGUMENT;
+    return plan_run(plan, dst_dev, count, false, stream, src_dev, sub_dev);
 }
 
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream) {

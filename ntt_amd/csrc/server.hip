@@ -254,6 +254,7 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
                 FusedJob& j = s.h_desc[nd++];
                 j = FusedJob{};
                 j.data = S->arena + q.off / sizeof(uint32_t);
+                j.src = j.data;
                 j.tw = inv ? q.plan->d_tw_inv : q.plan->d_tw_fwd;
                 j.lc = inv ? q.plan->d_lc_inv : q.plan->d_lc_fwd;
                 j.limbs = q.r.limbs;

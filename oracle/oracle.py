@@ -150,6 +150,21 @@ def inverse(A: np.ndarray, logN: int, moduli, domain: int = MONTGOMERY) -> np.nd
     return out
 
 
+def mod_down(ntt_out: np.ndarray, sub: np.ndarray, factors, moduli) -> np.ndarray:
+    """The subtract-and-scale epilogue of Cheddar's NTTForModDown
+    (src/NTT_cheddar.cu:573-599: res = src2 - NTT, then res * inv_p_prod mod
+    the limb's prime): out[l] = (sub[l] - ntt_out[l]) * factors[l] mod q_l,
+    canonical, for (limbs, N) arrays.  The padding multiply of src2
+    (579-584) is not restated (the build has no counterpart)."""
+    qs = np.array([int(q) for q, _ in moduli], dtype=np.int64)[:, None]
+    f = np.array([int(v) for v in factors], dtype=np.int64)[:, None] % qs
+    x = np.asarray(ntt_out, dtype=np.int64).reshape(len(moduli), -1)
+    y = np.asarray(sub, dtype=np.int64).reshape(len(moduli), -1)
+    d = (y - x) % qs                          # in [0, q) < 2^31
+    # d * f < 2^62: exact in int64
+    return (d * f % qs).astype(np.uint32)
+
+
 def fill_random_limbs(logN: int, moduli, seed: int) -> np.ndarray:
     """The reference's fill_random per limb, each limb re-seeded with `seed`."""
     qs, _ = _moduli_arrays(moduli)

@@ -245,3 +245,35 @@ def test_planted_mistake_missing_normalisation_is_caught(oracle_mod):
     b = list(range(1 << logN))
     got = oracle_mod.cpu_intt_butterfly(np.array(b, dtype=np.uint32), logN, q, itw, 1)
     assert got.tolist() != mathref.negacyclic_interp_bitrev(b, q, rt["psi"])
+
+
+def _mod_down_case(oracle_mod, logN=10):
+    """RNS exact division, the identity ModDown rests on: with P coprime to
+    every q and f = P^-1 mod q, (NTT(x + P y) - NTT(x)) * f = NTT(y)."""
+    moduli = DEFAULT_MODULI[:6]
+    P = 1000003 * 999983                       # two primes outside the moduli
+    rng = np.random.default_rng(11)
+    qs = [q for q, _ in moduli]
+    x = np.array([rng.integers(0, q, 1 << logN) for q in qs], dtype=np.uint32)
+    y = np.array([rng.integers(0, q, 1 << logN) for q in qs], dtype=np.uint32)
+    xpy = np.array([(x[l].astype(object) + P * y[l].astype(object)) % q for l, q in enumerate(qs)],
+                   dtype=np.uint32)
+    f = [pow(P, -1, q) for q in qs]
+    mont = oracle_mod.MONTGOMERY
+    return (oracle_mod.forward(x, logN, moduli, mont), oracle_mod.forward(xpy, logN, moduli, mont), f, moduli,
+            oracle_mod.forward(y, logN, moduli, mont))
+
+
+def test_mod_down_exact_division(oracle_mod):
+    ntt_x, sub, f, moduli, ntt_y = _mod_down_case(oracle_mod)
+    assert np.array_equal(oracle_mod.mod_down(ntt_x, sub, f, moduli), ntt_y)
+    # f = 1 is the plain difference, checked against Python integers
+    d = oracle_mod.mod_down(ntt_x, sub, [1] * len(moduli), moduli)
+    l, i, q = 3, 77, moduli[3][0]
+    assert int(d[l, i]) == (int(sub[l, i]) - int(ntt_x[l, i])) % q
+
+
+def test_planted_mistake_mod_down_operand_order_is_caught(oracle_mod):
+    """NTT - sub instead of sub - NTT (a sign slip) gives NTT(-y), not NTT(y)."""
+    ntt_x, sub, f, moduli, ntt_y = _mod_down_case(oracle_mod)
+    assert not np.array_equal(oracle_mod.mod_down(sub, ntt_x, f, moduli), ntt_y)
