@@ -452,6 +452,28 @@ def main():
             "vs_rtx4090_v04_ntt_reference_style": round(bfly_per_transform / (ref_ms * 1e-3) / 1e9
                                                         / RTX4090_V04_NTT_GBFLY, 3),
         })
+        # fused mod-down form over the whole batch: dst = (sub - NTT(src)) * f, in place
+        # on the pool; algorithmic bytes 12 per word (src and sub read once, dst written once)
+        sub = dev.DeviceBuffer(4 * words_per_op * B)
+        dev.fill_uniform(sub, qdev, limbs, logN, B, seed=1)
+        plan.set_mod_down_factors([3 + l for l in range(limbs)])
+        md = {}
+        for name, fn in (("forward", lambda: plan.forward(pool, B, stream)),
+                         ("mod_down", lambda: plan.forward_mod_down(pool, pool, sub, B, stream))):
+            fn()
+            ev_a.record(stream)
+            for _ in range(5):
+                fn()
+            ev_b.record(stream)
+            ev_b.synchronize()
+            md[name] = ev_a.elapsed_ms(ev_b) / 5
+        plan.set_mod_down_factors(None)
+        del sub
+        extras["mod_down"] = {
+            "ms": round(md["mod_down"], 4), "forward_ms": round(md["forward"], 4),
+            "gbfly_s": round(B * bfly_per_transform / (md["mod_down"] * 1e-3) / 1e9, 1),
+            "algorithmic_gb_s": round(12 * B * words_per_op / (md["mod_down"] * 1e-3) / 1e9, 1),
+            "cost_vs_forward": round(md["mod_down"] / md["forward"], 3)}
         log("extras: " + json.dumps(extras))
 
     mixed = None
