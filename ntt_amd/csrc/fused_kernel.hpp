@@ -198,6 +198,35 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[1 << EB], uint32_t* L, ui
     });
 }
 
+// Every field of a job descriptor, and of the limb constants, is the same in
+// all lanes (a work item is wave-uniform), but a descriptor read from memory,
+// or merged with one from the kernel arguments, lands in VGPRs.  Moving the
+// fields to SGPRs keeps pointers out of VGPRs (global accesses then take an
+// SGPR base plus a 32-bit VGPR offset) and frees registers for the words.
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+    const uint64_t v = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return reinterpret_cast<T*>(((uint64_t)hi << 32) | lo);
+}
+
+__device__ __forceinline__ FusedJob uniform_job(const FusedJob& j) {
+    FusedJob u;
+    u.data = uniform_ptr(j.data);
+    u.src = uniform_ptr(j.src);
+    u.sub = uniform_ptr(j.sub);
+    u.post = uniform_ptr(j.post);
+    u.tw = uniform_ptr(j.tw);
+    u.lc = uniform_ptr(j.lc);
+    u.limbs = __builtin_amdgcn_readfirstlane(j.limbs);
+    u.period = __builtin_amdgcn_readfirstlane(j.period);
+    u.item_begin = __builtin_amdgcn_readfirstlane(j.item_begin);
+    u.n = j.n;
+    u.inverse = j.inverse;
+    return u;
+}
+
 template <int n, bool INV, bool LAZY, int EB, bool MD>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds) {
     constexpr int E = 1 << EB;
@@ -218,7 +247,14 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     const uint32_t per = lsafe % job.period;
     gcu32* In = (gcu32*)(job.src + ((size_t)lsafe << n));
     gcu32* tw = (gcu32*)job.tw[per];
-    const LimbConst c = job.lc[per];
+    LimbConst c = job.lc[per];
+    if constexpr (G_BITS >= 6) {   // one limb per wave: the limb's table and constants are wave-uniform
+        tw = uniform_ptr(tw);
+        c.q = __builtin_amdgcn_readfirstlane(c.q);
+        c.q_inv = __builtin_amdgcn_readfirstlane(c.q_inv);
+        c.scale = __builtin_amdgcn_readfirstlane(c.scale);
+        c.flags = __builtin_amdgcn_readfirstlane(c.flags);
+    }
     uint32_t* L = lds + slot * STRIDE;
 
     uint32_t v[E];
@@ -270,7 +306,8 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         }
         if constexpr (MD) {
             // mod-down: (sub - NTT) * post, read at the positions stored below
-            const uint32_t f = job.post[per];
+            uint32_t f = job.post[per];
+            if constexpr (G_BITS >= 6) f = __builtin_amdgcn_readfirstlane(f);
             gcu32* Sb = (gcu32*)(job.sub + ((size_t)lsafe << n)) + ubase<LOL, EB>(opaque(u));
             if constexpr (LOL == 0) {
                 static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
@@ -343,6 +380,7 @@ __global__ __launch_bounds__(1 << (16 - EB), 1) void fused_kernel(FusedParams p)
             }
             job = p.jobs[lo];
         }
+        job = fused::uniform_job(job);
         fused::run_item<n, INV, LAZY, EB, MD>(job, (item - job.item_begin) << (16 - n), lds);
     }
 }
