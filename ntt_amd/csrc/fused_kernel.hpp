@@ -122,10 +122,16 @@ __device__ __forceinline__ uint32_t ubase(uint32_t u) {
 }
 
 // LDS word of word i in an exchange split on bit S: i without bit S
-// ("compress"), plus one padding word per 32.  Both steps distribute over
-// sums of disjoint bit fields, so the position of a word is
-// pad(compress(thread part)) + pad(compress(register part)).
-__host__ __device__ constexpr uint32_t lds_pad(uint32_t c) { return c + (c >> 5); }
+// ("compress"), plus one padding word per 2^PAD_SHIFT.  Both steps distribute
+// over sums of disjoint bit fields, so the position of a word is
+// pad(compress(thread part)) + pad(compress(register part)).  The padding
+// keeps the lanes of each exchange access on distinct banks: one word per 32
+// for window width 6, one per 64 for width 7 (one per 32 puts the lanes of
+// width-7 exchanges on 2-4 way conflicting banks).
+__host__ __device__ constexpr int pad_shift(int eb) { return eb == 7 ? 6 : 5; }
+
+template <int EB>
+__host__ __device__ constexpr uint32_t lds_pad(uint32_t c) { return c + (c >> pad_shift(EB)); }
 
 template <int S>
 __host__ __device__ constexpr uint32_t compress(uint32_t i) {
@@ -134,11 +140,11 @@ __host__ __device__ constexpr uint32_t compress(uint32_t i) {
 
 template <int S, int LO, int EB>
 __device__ __forceinline__ uint32_t lds_ubase(uint32_t u) {
-    return lds_pad(compress<S>(ubase<LO, EB>(u)));
+    return lds_pad<EB>(compress<S>(ubase<LO, EB>(u)));
 }
 
-template <int S, int LO, int X>
-inline constexpr uint32_t LDS_OFF = lds_pad(compress<S>((uint32_t)X << LO));
+template <int S, int LO, int X, int EB>
+inline constexpr uint32_t LDS_OFF = lds_pad<EB>(compress<S>((uint32_t)X << LO));
 
 template <int n, bool INV, bool LAZY, int K, int EB>
 __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict__ tw, uint32_t u, uint32_t q,
@@ -184,14 +190,14 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[1 << EB], uint32_t* L, ui
         if (valid) {
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
-                if constexpr (((x >> JA) & 1) == h) LA[LDS_OFF<S, LOA, x>] = v[REG<n, INV, K, x, EB>];
+                if constexpr (((x >> JA) & 1) == h) LA[LDS_OFF<S, LOA, x, EB>] = v[REG<n, INV, K, x, EB>];
             });
         }
         __syncthreads();
         if (valid) {
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
-                if constexpr (((x >> JB) & 1) == h) v[REG<n, INV, K + 1, x, EB>] = LB[LDS_OFF<S, LOB, x>];
+                if constexpr (((x >> JB) & 1) == h) v[REG<n, INV, K + 1, x, EB>] = LB[LDS_OFF<S, LOB, x, EB>];
             });
         }
         __syncthreads();
@@ -231,7 +237,7 @@ template <int n, bool INV, bool LAZY, int EB, bool MD>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds) {
     constexpr int E = 1 << EB;
     constexpr int G_BITS = n - EB;
-    constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 6));
+    constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 1 - pad_shift(EB)));
     constexpr int NR = nrounds(n, EB);
     constexpr int LO0 = win_lo(n, INV, 0, EB), LOL = win_lo(n, INV, NR - 1, EB);
     const uint32_t t = threadIdx.x;

@@ -88,9 +88,15 @@ def compress(i: int, s: int) -> int:
     return ((i >> (s + 1)) << s) | (i & ((1 << s) - 1))
 
 
-def lds_slot_stride(n: int) -> int:
+def pad_shift(eb: int) -> int:
+    """One LDS padding word per 2^pad_shift words (fused_kernel.hpp)."""
+    return 6 if eb == 7 else 5
+
+
+def lds_slot_stride(n: int, eb: int = None) -> int:
+    eb = E_BITS if eb is None else eb
     half = 1 << (n - 1)
-    return half + (half >> 5)
+    return half + (half >> pad_shift(eb))
 
 
 def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False, eb: int = None):
@@ -111,7 +117,8 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False, eb: int 
     A = [list(a) for a in limbs_data]
     ws, sb, pm, sg = windows(n, inv, eb), stage_bits(n, inv, eb), perms(n, inv, eb), sigmas(n, inv, eb)
     assert sum(len(s) for s in sb) == n
-    stride = lds_slot_stride(n)
+    stride = lds_slot_stride(n, eb)
+    ps = pad_shift(eb)
     lds_words = lpw * stride
     assert lds_words <= 33 * 1024 + 64
     regs = [[None] * E for _ in range(THREADS)]
@@ -177,7 +184,7 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False, eb: int 
                     i = vidx(lo, u, xw, eb)
                     assert ((i >> s) & 1) == h
                     c = compress(i, s)
-                    a = slot * stride + c + (c >> 5)
+                    a = slot * stride + c + (c >> ps)
                     assert lds[a] is None, "two writers of one LDS word"
                     lds[a] = regs[t][reg_of(p, xw)]
                     written[t].add(reg_of(p, xw))
@@ -192,7 +199,7 @@ def run(limbs_data, tws, qs, n: int, inv: bool, scale=None, lazy=False, eb: int 
                     assert r in written[t], "register read before its value was saved"
                     i = vidx(lo_b, u, xw, eb)
                     c = compress(i, s)
-                    a = slot * stride + c + (c >> 5)
+                    a = slot * stride + c + (c >> ps)
                     assert lds[a] is not None, "read of an unwritten LDS word"
                     regs[t][r] = lds[a]
     lo = ws[-1]
