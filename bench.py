@@ -474,6 +474,22 @@ def main():
             "gbfly_s": round(B * bfly_per_transform / (md["mod_down"] * 1e-3) / 1e9, 1),
             "algorithmic_gb_s": round(12 * B * words_per_op / (md["mod_down"] * 1e-3) / 1e9, 1),
             "cost_vs_forward": round(md["mod_down"] / md["forward"], 3)}
+        # fused engine with on-the-fly twiddles in the low-bit round, against the table
+        if engine_used == "fused":
+            ot = {}
+            for mode, tag in ((ntt_amd.TWIDDLE_TABLE, "table"), (ntt_amd.TWIDDLE_OT, "ot")):
+                plan.set_twiddle_mode(mode)
+                for name, fn in (("forward", lambda: plan.forward(pool, B, stream)),
+                                 ("inverse", lambda: plan.inverse(pool, B, stream))):
+                    fn()
+                    ev_a.record(stream)
+                    for _ in range(5):
+                        fn()
+                    ev_b.record(stream)
+                    ev_b.synchronize()
+                    ot[f"{tag}_{name}_ms"] = round(ev_a.elapsed_ms(ev_b) / 5, 4)
+            plan.set_twiddle_mode(ntt_amd.TWIDDLE_TABLE)
+            extras["twiddle_ot"] = ot
         log("extras: " + json.dumps(extras))
 
     mixed = None

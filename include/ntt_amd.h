@@ -55,6 +55,13 @@ typedef enum { NTT_FORWARD = 0, NTT_INVERSE = 1 } ntt_direction;
  * AUTO: the autotuned choice for the launch size, else FUSED where supported. */
 typedef enum { NTT_ENGINE_AUTO = 0, NTT_ENGINE_TWO_PASS = 1, NTT_ENGINE_FUSED = 2 } ntt_engine;
 
+/* Where the fused engine takes twiddles in the round whose threads each need
+ * their own (the upper levels of the table).  TABLE: from the table.  OT: on
+ * the fly, tw[i] = mont_mul(tw[i mod 64], msb[i / 64]), from the table's
+ * first 64 words and an N/64-word appendix msb[h] = tw[64 h] that the
+ * context stores after every table of 2^10 words or more. */
+typedef enum { NTT_TWIDDLE_TABLE = 0, NTT_TWIDDLE_OT = 1 } ntt_twiddle_mode;
+
 /* ModulusConfig, src/ntt_config.h:6-9 */
 typedef struct {
     uint32_t qi;
@@ -203,6 +210,15 @@ int ntt_plan_set_constants(ntt_plan* plan, const uint32_t* prologue, const uint3
  * 128 words each, 6 = 1024 threads x 64 words, 0 = the tuned choice
  * (default: 7 forward, 6 inverse). */
 int ntt_plan_set_fused_shape(ntt_plan* plan, int eb);
+/* On-the-fly twiddles (ntt_twiddle_mode) for the plan's fused-engine launches:
+ * the compressed-twiddle scheme of Cheddar (msb table of every LsbSize-th
+ * twiddle, NTT_cheddar.cu:1283-1289; MultiRadixNTT_OT / MultiRadixINTT_OT in
+ * the last stage group, 79-93 and 421-435).  A fused launch in this mode
+ * reads 2^10 + N/64 table words per limb in place of N, at one Montgomery
+ * multiply per twiddle of that round; shapes are fixed (forward 7 lazy when
+ * every q < 2^30, else 6 canonical; inverse 6 canonical).  Two-pass launches
+ * and the mod-down form read the table as before. */
+int ntt_plan_set_twiddle_mode(ntt_plan* plan, int mode);
 /* Arithmetic of the fused forward kernel: 1 = lazy reduction (used only when
  * every q < 2^30), 0 = canonical, -1 = the tuned choice (default lazy).  The
  * inverse fused kernel is always canonical.  Outputs are canonical either way. */

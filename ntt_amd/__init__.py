@@ -45,6 +45,7 @@ STATUS = {
 PLAIN, MONTGOMERY = 0, 1
 FORWARD, INVERSE = 0, 1
 ENGINE_AUTO, ENGINE_TWO_PASS, ENGINE_FUSED = 0, 1, 2
+TWIDDLE_TABLE, TWIDDLE_OT = 0, 1
 
 
 class NTTError(RuntimeError):
@@ -213,6 +214,7 @@ _PROTOS = {
     "ntt_plan_set_engine": (_I, [_P, _I]),
     "ntt_plan_set_fused_shape": (_I, [_P, _I]),
     "ntt_plan_set_fused_arith": (_I, [_P, _I]),
+    "ntt_plan_set_twiddle_mode": (_I, [_P, _I]),
     "ntt_plan_set_constants": (_I, [_P, _PU32, _PU32]),
     "ntt_plan_get_fused_variant": (_I, [_P, _SZ, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ntt_plan_get_engine": (_I, [_P, _SZ]),
@@ -685,6 +687,10 @@ class Plan:
         p, e = arr(prologue), arr(epilogue)
         _check(_lib.ntt_plan_set_constants(self.handle, p[0] if p else None, e[0] if e else None),
                "ntt_plan_set_constants")
+
+    def set_twiddle_mode(self, mode: int):
+        """TWIDDLE_TABLE (default) or TWIDDLE_OT: on-the-fly twiddles in the fused engine."""
+        _check(_lib.ntt_plan_set_twiddle_mode(self.handle, mode), "ntt_plan_set_twiddle_mode")
 
     def set_fused_arith(self, lazy: int):
         """Fused forward arithmetic: 1 lazy (when every q < 2^30), 0 canonical, -1 tuned."""

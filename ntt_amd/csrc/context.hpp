@@ -75,6 +75,7 @@ struct ntt_plan {
     std::vector<uint32_t> h_scale_fwd, h_scale_inv;    // host copies of the scale words
     bool prologue = false;                         // forward input multiplied by per-limb constants
     uint32_t* d_post = nullptr;                    // mod-down multipliers to_mont(f_l), own allocation
+    bool twiddle_ot = false;                       // fused engine: on-the-fly twiddles (NTT_TWIDDLE_OT)
 };
 
 namespace ntt {

@@ -36,6 +36,10 @@ uint32_t fused_items(int logN, size_t limbs) {
 FusedFn fused_fn_eb6(int n, bool inv, bool lazy);   // fused_eb6.hip
 FusedFn fused_fn_eb7(int n, bool inv, bool lazy);   // fused_eb7.hip
 FusedFn fused_fn_md(int n, bool lazy);               // fused_md.hip
+FusedFn fused_fn_ot(int n, bool inv, bool lazy);     // fused_ot.hip
+
+// Window width of the on-the-fly twiddle kernels.
+static int ot_eb(bool inv, bool lazy) { return !inv && lazy ? 7 : 6; }
 
 // Window width of the mod-down kernels: 7 with lazy arithmetic, 6 canonical.
 static int md_eb(bool lazy) { return lazy ? 7 : 6; }
@@ -54,6 +58,13 @@ int fused_prepare() {
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
             for (int d = 0; d < 8 && attr_err == hipSuccess; ++d) {
                 FusedFn f = fused_fn(m, d & 1, 6 + ((d >> 1) & 1), d >= 4);
+                if (f)
+                    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            }
+        for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
+            for (int d = 0; d < 3 && attr_err == hipSuccess; ++d) {
+                FusedFn f = fused_fn_ot(m, d == 2, d == 1);
                 if (f)
                     attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -123,6 +134,14 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.total_items = fused_items(P->logN, limbs);
     p.counter = nullptr;
     p.stagger = stagger < 0 ? 0u : (uint32_t)stagger;
+    if (P->twiddle_ot && !sub) {
+        // on-the-fly twiddles: fixed shapes, lazy forward when every q < 2^30
+        const bool lz = P->lazy && !inverse;
+        FusedFn fn = fused_fn_ot(P->logN, inverse, lz);
+        if (!fn) return NTT_ERR_CONFIG;
+        if (p.total_items == 0) return NTT_OK;
+        return launch_fused_fn(fn, ot_eb(inverse, lz), p, stream, 0);
+    }
     if (sub) {
         const bool lz = P->lazy;   // the mod-down shapes: lazy when every q < 2^30
         FusedFn fn = fused_fn_md(P->logN, lz);

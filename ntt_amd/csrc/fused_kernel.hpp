@@ -146,7 +146,11 @@ __device__ __forceinline__ uint32_t lds_ubase(uint32_t u) {
 template <int S, int LO, int X, int EB>
 inline constexpr uint32_t LDS_OFF = lds_pad<EB>(compress<S>((uint32_t)X << LO));
 
-template <int n, bool INV, bool LAZY, int K, int EB>
+// OT: in the round whose window is the low bits (LO == 0: every thread needs
+// its own twiddles), each twiddle is formed on the fly from the table's first
+// 64 words and its msb appendix (FusedJob tables, fused_types.hpp) instead of
+// being read from the table's upper levels.
+template <int n, bool INV, bool LAZY, int K, int EB, bool OT = false>
 __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict__ tw, uint32_t u, uint32_t q,
                                        uint32_t q_inv) {
     constexpr int E = 1 << EB;
@@ -156,12 +160,26 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
     static_for<EB>([&](auto JJ) __attribute__((always_inline)) {
         constexpr int j = INV ? decltype(JJ)::value : EB - 1 - decltype(JJ)::value;
         if constexpr (j >= JLO && j <= JHI) {
+            // on-the-fly twiddles of this stage: index (TB << k) + c with
+            // c < 2^k, k <= 6, splits as lo = (TB mod 2^(6-k)) << k + c and
+            // hi = TB >> (6 - k); msb[hi] is one word per thread per stage
+            constexpr int k = EB - 1 - j;
+            constexpr bool OTS = OT && LO == 0 && k <= OT_LSB_BITS;
+            uint32_t ot_msb = 0, ot_lo = 0;
+            if constexpr (OTS) {
+                ot_msb = tw[(1u << n) + (TB >> (OT_LSB_BITS - k))];
+                ot_lo = (TB & ((1u << (OT_LSB_BITS - k)) - 1)) << k;
+            }
             static_for<E / 2>([&](auto P) __attribute__((always_inline)) {
                 // P enumerates the words with bit j clear: x = P with a 0 inserted at bit j
                 constexpr int p = decltype(P)::value;
                 constexpr int x = ((p >> j) << (j + 1)) | (p & ((1 << j) - 1));
                 constexpr int a = REG<n, INV, K, x, EB>, b = REG<n, INV, K, x | (1 << j), EB>;
-                const uint32_t w = tw[(TB << (EB - 1 - j)) + (uint32_t)(x >> (j + 1))];
+                uint32_t w;
+                if constexpr (OTS)
+                    w = mont_mul(tw[ot_lo + (uint32_t)(x >> (j + 1))], ot_msb, q, q_inv);
+                else
+                    w = tw[(TB << (EB - 1 - j)) + (uint32_t)(x >> (j + 1))];
                 if (LAZY) {
                     if (INV)
                         gs_butterfly_lazy(v[a], v[b], w, q, q_inv);
@@ -233,7 +251,7 @@ __device__ __forceinline__ FusedJob uniform_job(const FusedJob& j) {
     return u;
 }
 
-template <int n, bool INV, bool LAZY, int EB, bool MD>
+template <int n, bool INV, bool LAZY, int EB, bool MD, bool OT>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds) {
     constexpr int E = 1 << EB;
     constexpr int G_BITS = n - EB;
@@ -292,12 +310,12 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         for (int x = 0; x < E; ++x) v[x] = 0;
     }
 
-    stages<n, INV, LAZY, 0, EB>(v, tw, u, c.q, c.q_inv);
+    stages<n, INV, LAZY, 0, EB, OT>(v, tw, u, c.q, c.q_inv);
     exchange<n, INV, 0, EB>(v, L, u, valid);
-    stages<n, INV, LAZY, 1, EB>(v, tw, u, c.q, c.q_inv);
+    stages<n, INV, LAZY, 1, EB, OT>(v, tw, u, c.q, c.q_inv);
     if constexpr (NR == 3) {
         exchange<n, INV, 1, EB>(v, L, u, valid);
-        stages<n, INV, LAZY, 2, EB>(v, tw, u, c.q, c.q_inv);
+        stages<n, INV, LAZY, 2, EB, OT>(v, tw, u, c.q, c.q_inv);
     }
 
     // ---- store (last round layout) ----
@@ -357,7 +375,8 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 // an item is 2^(16-n) consecutive limbs of one job.  Every thread of the
 // workgroup runs every barrier of every item.
 // MD: forward mod-down form, storing (sub - NTT) * post (FusedJob).
-template <int n, bool INV, bool LAZY, int EB, bool MD = false>
+// OT: on-the-fly twiddles in the low-bit round (stages).
+template <int n, bool INV, bool LAZY, int EB, bool MD = false, bool OT = false>
 __global__ __launch_bounds__(1 << (16 - EB), 1) void fused_kernel(FusedParams p) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_next;
@@ -387,7 +406,7 @@ __global__ __launch_bounds__(1 << (16 - EB), 1) void fused_kernel(FusedParams p)
             job = p.jobs[lo];
         }
         job = fused::uniform_job(job);
-        fused::run_item<n, INV, LAZY, EB, MD>(job, (item - job.item_begin) << (16 - n), lds);
+        fused::run_item<n, INV, LAZY, EB, MD, OT>(job, (item - job.item_begin) << (16 - n), lds);
     }
 }
 

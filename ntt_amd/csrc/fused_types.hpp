@@ -2,6 +2,7 @@
 // the host code that builds them.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 
 namespace ntt {
@@ -13,6 +14,13 @@ constexpr uint32_t FUSED_LDS_WORDS = 32768 + 1024;
 // autotuner overrides them per launch size.
 constexpr int FUSED_DEFAULT_EB_FWD = 7;
 constexpr int FUSED_DEFAULT_EB_INV = 6;
+
+// On-the-fly twiddles: a cached table of N = 2^n words (n >= 10) is followed
+// by N / 64 words msb[h] = tw[64 h], so tw[i] = mont_mul(tw[i % 64], msb[i / 64])
+// (the exponent brv(i) splits over the disjoint bit fields of i).
+constexpr int OT_LSB_BITS = 6;
+constexpr int OT_MIN_LOG_N = 10;
+inline size_t ot_msb_words(int logN) { return logN >= OT_MIN_LOG_N ? ((size_t)1 << logN) >> OT_LSB_BITS : 0; }
 
 struct FusedParams;
 using FusedFn = void (*)(FusedParams);

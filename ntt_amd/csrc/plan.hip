@@ -31,17 +31,18 @@ int TwiddleCache::get(uint32_t q, uint32_t psi_root, int logN, bool inverse, con
         *out = it->second;
         return NTT_OK;
     }
-    const size_t N = (size_t)1 << logN;
-    std::vector<uint32_t> host(N);
+    const size_t N = (size_t)1 << logN, M = ot_msb_words(logN);
+    std::vector<uint32_t> host(N + M);
     twiddles_bitrev(logN, q, psi_root, true, host.data());
+    for (size_t h = 0; h < M; ++h) host[N + h] = host[h << OT_LSB_BITS];   // on-the-fly appendix
     uint32_t* d = nullptr;
-    if (hipMalloc(&d, N * sizeof(uint32_t)) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
-    if (hipMemcpy(d, host.data(), N * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMalloc(&d, (N + M) * sizeof(uint32_t)) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
+    if (hipMemcpy(d, host.data(), (N + M) * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
         (void)hipFree(d);
         return NTT_ERR_HIP;
     }
     tables[key] = d;
-    bytes += N * sizeof(uint32_t);
+    bytes += (N + M) * sizeof(uint32_t);
     *out = d;
     return NTT_OK;
 }
@@ -176,6 +177,10 @@ void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, int* eb, 
     const bool want_lazy = P->fused_arith >= 0 ? P->fused_arith == 1 : (tuned ? t.fused_lazy : true);
     *lazy = want_lazy && P->lazy && !inverse;   // lazy kernels: forward only, every q < 2^30
     *stagger = tuned ? t.stagger : 0;
+    if (P->twiddle_ot) {   // fixed shapes of the on-the-fly twiddle kernels
+        *lazy = P->lazy && !inverse;
+        *eb = *lazy ? 7 : 6;
+    }
 }
 
 int log2_bucket(size_t v) {
@@ -337,6 +342,12 @@ int ntt_plan_set_constants(ntt_plan* plan, const uint32_t* prologue, const uint3
     plan->prologue = prologue != nullptr;
     plan->h_scale_fwd = sf;
     plan->h_scale_inv = si;
+    return NTT_OK;
+}
+
+int ntt_plan_set_twiddle_mode(ntt_plan* plan, int mode) {
+    if (!plan || (mode != NTT_TWIDDLE_TABLE && mode != NTT_TWIDDLE_OT)) return NTT_ERR_INVALID_ARGUMENT;
+    plan->twiddle_ot = mode == NTT_TWIDDLE_OT;
     return NTT_OK;
 }
 

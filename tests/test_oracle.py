@@ -277,3 +277,27 @@ def test_planted_mistake_mod_down_operand_order_is_caught(oracle_mod):
     """NTT - sub instead of sub - NTT (a sign slip) gives NTT(-y), not NTT(y)."""
     ntt_x, sub, f, moduli, ntt_y = _mod_down_case(oracle_mod)
     assert not np.array_equal(oracle_mod.mod_down(sub, ntt_x, f, moduli), ntt_y)
+
+
+def test_twiddle_split_identity(oracle_mod):
+    """The identity the on-the-fly twiddles rest on: with tw[i] = psi^brv(i),
+    tw[i] = tw[i mod 64] * tw[64 (i div 64)] mod q for every i (brv splits over
+    the disjoint bit fields of i), for the forward and the inverse table."""
+    for logN, (q, g) in ((10, (12289, 11)), (13, (998244353, 3)), (16, (1073479681, 11))):
+        rc, rt = oracle_mod.make_modulus_runtime(logN, q, g)
+        assert rc == 0
+        for root in (rt["psi"], rt["psi_inv"]):
+            tw = oracle_mod.twiddle_table(logN, q, root).astype(np.uint64)
+            i = np.arange(1 << logN)
+            assert np.array_equal(tw, tw[i & 63] * tw[(i >> 6) << 6] % q)
+
+
+def test_planted_mistake_twiddle_split_is_caught(oracle_mod):
+    """An msb table indexed tw[i div 64] instead of tw[64 (i div 64)] fails."""
+    logN, q, g = 11, 12289, 11
+    rc, rt = oracle_mod.make_modulus_runtime(logN, q, g)
+    assert rc == 0
+    tw = oracle_mod.twiddle_table(logN, q, rt["psi"]).astype(np.uint64)
+    i = np.arange(1 << logN)
+    assert np.array_equal(tw, tw[i & 63] * tw[(i >> 6) << 6] % q)
+    assert not np.array_equal(tw, tw[i & 63] * tw[i >> 6] % q)
