@@ -268,6 +268,34 @@ int ntt_plan_run_pass(ntt_plan* plan, uint32_t* A_dev, size_t count, int directi
                       void* stream);
 
 /* ---------------------------------------------------------------------------
+ * 64-bit words: the word width of Cheddar's NTTHandler<uint64_t>
+ * (NTT_cheddar.cu:1313) for primes beyond 2^31; the reference's own kernels
+ * are 32-bit.  Primes q < 2^62 with q = 1 mod 2N; `root` is any quadratic
+ * non-residue mod q (a primitive root is one), psi = root^((q-1)/2N).
+ * Layout A[limb * N + i] of uint64 words, in place.  Normal domain in and
+ * out: forward maps natural-order coefficients to the bit-reversed
+ * evaluations at psi^(2 brv(k) + 1); inverse maps back, N^-1 included.
+ * 2 <= logN <= 17.  Two passes (column, row) of LDS-staged radix-2 stages.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    uint64_t q;
+    uint64_t root;
+} ntt_modulus_config64;
+typedef struct ntt_plan64 ntt_plan64;
+/* The `count` largest primes q < min(bound, 2^62) with q = 1 mod 2^(logN+1),
+ * descending, each with its smallest quadratic non-residue.  Returns the
+ * number written. */
+int ntt_generate_moduli64(int logN, uint64_t bound, int count, ntt_modulus_config64* out);
+/* Builds and uploads the plan's Montgomery-form (R = 2^64) tables; checks
+ * every modulus like ntt_plan_create. */
+int ntt_plan64_create(ntt_context* ctx, int logN, const ntt_modulus_config64* moduli, size_t limbs,
+                      ntt_plan64** out);
+int ntt_plan64_destroy(ntt_plan64* plan);
+/* `count` operands back to back (operand k at A_dev + k * limbs * N). */
+int ntt_plan64_forward(ntt_plan64* plan, uint64_t* A_dev, size_t count, void* stream);
+int ntt_plan64_inverse(ntt_plan64* plan, uint64_t* A_dev, size_t count, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Autotuner.  No counterpart in the reference, which fixes its phase split at
  * compile time (DefaultNTTLaunchConfig<LogN>, src/ntt_config.h:62-73).
  * Measures every valid (radix split, stage_merging, warp_batching) per pass

@@ -70,6 +70,18 @@ class ModulusConfig(ctypes.Structure):
         yield self.primitive_root
 
 
+class ModulusConfig64(ctypes.Structure):
+    """ntt_modulus_config64{q, root}: a prime q < 2^62 and a quadratic non-residue."""
+    _fields_ = [("q", ctypes.c_uint64), ("root", ctypes.c_uint64)]
+
+    def __repr__(self):
+        return f"ModulusConfig64({self.q}, {self.root})"
+
+    def __iter__(self):
+        yield self.q
+        yield self.root
+
+
 class PhaseConfig(ctypes.Structure):
     """PhaseConfig{radix_stages, stage_merging, warp_batching}, src/ntt_config.h:12-16."""
     _fields_ = [("radix_stages", ctypes.c_int32), ("stage_merging", ctypes.c_int32),
@@ -206,6 +218,11 @@ _PROTOS = {
     "ntt_plan_inverse": (_I, [_P, _P, _P]),
     "ntt_plan_forward_many": (_I, [_P, _P, _SZ, _P]),
     "ntt_plan_inverse_many": (_I, [_P, _P, _SZ, _P]),
+    "ntt_generate_moduli64": (_I, [_I, ctypes.c_uint64, _I, ctypes.POINTER(ModulusConfig64)]),
+    "ntt_plan64_create": (_I, [_P, _I, ctypes.POINTER(ModulusConfig64), _SZ, ctypes.POINTER(_P)]),
+    "ntt_plan64_destroy": (_I, [_P]),
+    "ntt_plan64_forward": (_I, [_P, _P, _SZ, _P]),
+    "ntt_plan64_inverse": (_I, [_P, _P, _SZ, _P]),
     "ntt_plan_forward_to": (_I, [_P, _P, _P, _SZ, _P]),
     "ntt_plan_inverse_to": (_I, [_P, _P, _P, _SZ, _P]),
     "ntt_plan_set_mod_down_factors": (_I, [_P, _PU32]),
@@ -347,6 +364,16 @@ def generate_moduli(logN: int, count: int, bound: int = 1 << 30) -> List[Modulus
     return list(arr)[:n]
 
 
+def generate_moduli64(logN: int, count: int, bound: int = 1 << 62) -> List[ModulusConfig64]:
+    """The `count` largest primes q < bound (<= 2^62) with q = 1 mod 2N, each
+    with its smallest quadratic non-residue."""
+    arr = (ModulusConfig64 * max(count, 1))()
+    n = _lib.ntt_generate_moduli64(logN, bound, count, arr)
+    if n < 0:
+        raise NTTError(-n, "generate_moduli64")
+    return list(arr)[:n]
+
+
 def as_moduli(moduli) -> List[ModulusConfig]:
     return [m if isinstance(m, ModulusConfig) else ModulusConfig(int(m[0]), int(m[1])) for m in moduli]
 
@@ -452,6 +479,9 @@ class Context:
 
     def plan(self, logN: int, moduli, domain: int = MONTGOMERY, cfg: Optional[TransformConfig] = None) -> "Plan":
         return Plan(self, logN, moduli, domain, cfg)
+
+    def plan64(self, logN: int, moduli) -> "Plan64":
+        return Plan64(self, logN, moduli)
 
     def autotune(self, logN: int, moduli, count: int = 1, domain: int = MONTGOMERY, reps: int = 3) -> dict:
         """Measure phase configs for `count` operands of these moduli; the winner
@@ -610,6 +640,34 @@ class Batch:
     def close(self):
         if self.handle:
             _lib.ntt_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan64:
+    """64-bit word transforms (ntt_plan64_*): normal domain in and out."""
+
+    def __init__(self, ctx, logN: int, moduli):
+        ms = [m if isinstance(m, ModulusConfig64) else ModulusConfig64(int(m[0]), int(m[1])) for m in moduli]
+        arr = (ModulusConfig64 * len(ms))(*ms)
+        h = _P()
+        _check(_lib.ntt_plan64_create(ctx.handle, logN, arr, len(ms), ctypes.byref(h)), "ntt_plan64_create")
+        self.ctx, self.handle, self.logN, self.limbs = ctx, h, logN, len(ms)
+
+    def forward(self, A, count: int = 1, stream=None):
+        _check(_lib.ntt_plan64_forward(self.handle, _addr(A), count, _stream(stream)), "ntt_plan64_forward")
+
+    def inverse(self, A, count: int = 1, stream=None):
+        _check(_lib.ntt_plan64_inverse(self.handle, _addr(A), count, _stream(stream)), "ntt_plan64_inverse")
+
+    def close(self):
+        if self.handle:
+            _lib.ntt_plan64_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

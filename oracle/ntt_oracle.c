@@ -318,3 +318,90 @@ int oracle_fill_random_limbs(uint32_t* A, int logN, const uint32_t* qs, size_t l
     }
     return 0;
 }
+
+/* ---- 64-bit words --------------------------------------------------------
+ * The same textbook transforms (cpu_ntt_butterfly / cpu_intt_butterfly,
+ * src/v04_montgomery.cu:742-784) over 64-bit words and primes q < 2^62, the
+ * word width of Cheddar's NTTHandler<uint64_t> (src/NTT_cheddar.cu:1313),
+ * which has no CPU path of its own.  Normal domain throughout: the forward
+ * maps natural-order coefficients to the bit-reversed evaluations at
+ * psi^(2 brv(k) + 1); the inverse undoes it including the N^-1 scaling.
+ * psi = root^((q-1) / 2N) must be a primitive 2N-th root (psi^N = q - 1).
+ * Pinned in tests/test_oracle.py: equals the 32-bit oracle for q < 2^31,
+ * brute-force evaluation with Python integers, roundtrip. */
+typedef unsigned __int128 u128;
+
+uint64_t oracle_mul_mod64(uint64_t a, uint64_t b, uint64_t q) { return (uint64_t)((u128)a * b % q); }
+
+uint64_t oracle_pow_mod64(uint64_t b, uint64_t e, uint64_t q) {
+    uint64_t r = 1 % q;
+    b %= q;
+    while (e) {
+        if (e & 1) r = oracle_mul_mod64(r, b, q);
+        b = oracle_mul_mod64(b, b, q);
+        e >>= 1;
+    }
+    return r;
+}
+
+/* psi for (q, root) at logN; 0 if q != 1 mod 2N or psi is not primitive. */
+uint64_t oracle_psi64(int logN, uint64_t q, uint64_t root) {
+    const uint64_t two_n = (uint64_t)2 << logN;
+    if (q < 3 || (q - 1) % two_n) return 0;
+    const uint64_t psi = oracle_pow_mod64(root, (q - 1) / two_n, q);
+    if (oracle_pow_mod64(psi, two_n >> 1, q) != q - 1) return 0;
+    return psi;
+}
+
+static void twiddles64(int logN, uint64_t q, uint64_t psi, uint64_t* tw) {
+    const uint32_t N = 1u << logN;
+    for (uint32_t i = 0; i < N; ++i) tw[i] = oracle_pow_mod64(psi, oracle_bit_reverse(i, logN), q);
+}
+
+int oracle_forward64(uint64_t* A, int logN, const uint64_t* qs, const uint64_t* roots, size_t limbs) {
+    const uint32_t N = 1u << logN;
+    uint64_t* tw = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    if (!tw) return 2;
+    for (size_t l = 0; l < limbs; ++l) {
+        const uint64_t q = qs[l], psi = oracle_psi64(logN, q, roots[l]);
+        if (!psi) { free(tw); return 1; }
+        twiddles64(logN, q, psi, tw);
+        uint64_t* a = A + l * N;
+        for (uint32_t m = 1, t = N >> 1; m < N; m <<= 1, t >>= 1)
+            for (uint32_t i = 0; i < m; ++i) {
+                const uint64_t w = tw[m + i];
+                for (uint32_t j = 2 * i * t; j < 2 * i * t + t; ++j) {
+                    const uint64_t u = a[j], v = oracle_mul_mod64(a[j + t], w, q);
+                    a[j] = (uint64_t)(((u128)u + v) % q);
+                    a[j + t] = (uint64_t)(((u128)u + q - v) % q);
+                }
+            }
+    }
+    free(tw);
+    return 0;
+}
+
+int oracle_inverse64(uint64_t* A, int logN, const uint64_t* qs, const uint64_t* roots, size_t limbs) {
+    const uint32_t N = 1u << logN;
+    uint64_t* tw = (uint64_t*)malloc(sizeof(uint64_t) * N);
+    if (!tw) return 2;
+    for (size_t l = 0; l < limbs; ++l) {
+        const uint64_t q = qs[l], psi = oracle_psi64(logN, q, roots[l]);
+        if (!psi) { free(tw); return 1; }
+        twiddles64(logN, q, oracle_pow_mod64(psi, q - 2, q), tw);
+        uint64_t* a = A + l * N;
+        for (uint32_t m = N >> 1, t = 1; m >= 1; m >>= 1, t <<= 1)
+            for (uint32_t i = 0; i < m; ++i) {
+                const uint64_t w = tw[m + i];
+                for (uint32_t j = 2 * i * t; j < 2 * i * t + t; ++j) {
+                    const uint64_t u = a[j], v = a[j + t];
+                    a[j] = (uint64_t)(((u128)u + v) % q);
+                    a[j + t] = oracle_mul_mod64((uint64_t)(((u128)u + q - v) % q), w, q);
+                }
+            }
+        const uint64_t ninv = oracle_pow_mod64(N, q - 2, q);
+        for (uint32_t j = 0; j < N; ++j) a[j] = oracle_mul_mod64(a[j], ninv, q);
+    }
+    free(tw);
+    return 0;
+}

@@ -23,6 +23,7 @@ _lib = None
 U32 = ctypes.c_uint32
 U64 = ctypes.c_uint64
 P32 = ctypes.POINTER(ctypes.c_uint32)
+P64 = ctypes.POINTER(ctypes.c_uint64)
 
 
 class ModulusRuntime(ctypes.Structure):
@@ -70,6 +71,11 @@ def lib():
             ("oracle_fill_random_limbs", ctypes.c_int, [P32, ctypes.c_int, P32, ctypes.c_size_t, U32]),
             ("oracle_to_mont_inplace", None, [P32, ctypes.c_size_t, U32]),
             ("oracle_from_mont_inplace", None, [P32, ctypes.c_size_t, U32, U32]),
+            ("oracle_mul_mod64", U64, [U64, U64, U64]),
+            ("oracle_pow_mod64", U64, [U64, U64, U64]),
+            ("oracle_psi64", U64, [ctypes.c_int, U64, U64]),
+            ("oracle_forward64", ctypes.c_int, [P64, ctypes.c_int, P64, P64, ctypes.c_size_t]),
+            ("oracle_inverse64", ctypes.c_int, [P64, ctypes.c_int, P64, P64, ctypes.c_size_t]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -163,6 +169,38 @@ def mod_down(ntt_out: np.ndarray, sub: np.ndarray, factors, moduli) -> np.ndarra
     d = (y - x) % qs                          # in [0, q) < 2^31
     # d * f < 2^62: exact in int64
     return (d * f % qs).astype(np.uint32)
+
+
+def psi64(logN: int, q: int, root: int) -> int:
+    return int(lib().oracle_psi64(logN, q, root))
+
+
+def _moduli64(moduli):
+    qs = np.array([int(q) for q, _ in moduli], dtype=np.uint64)
+    rs = np.array([int(r) for _, r in moduli], dtype=np.uint64)
+    return qs, rs
+
+
+def forward64(A: np.ndarray, logN: int, moduli) -> np.ndarray:
+    """Normal-domain forward NTT over 64-bit words of a (limbs, N) array;
+    moduli = [(q, root)] with q < 2^62 (ntt_oracle.c, 64-bit section)."""
+    out = np.ascontiguousarray(A, dtype=np.uint64).copy()
+    qs, rs = _moduli64(moduli)
+    rc = lib().oracle_forward64(out.ctypes.data_as(P64), logN, qs.ctypes.data_as(P64), rs.ctypes.data_as(P64),
+                                len(moduli))
+    if rc:
+        raise ValueError(f"oracle_forward64 failed rc={rc}")
+    return out
+
+
+def inverse64(A: np.ndarray, logN: int, moduli) -> np.ndarray:
+    out = np.ascontiguousarray(A, dtype=np.uint64).copy()
+    qs, rs = _moduli64(moduli)
+    rc = lib().oracle_inverse64(out.ctypes.data_as(P64), logN, qs.ctypes.data_as(P64), rs.ctypes.data_as(P64),
+                                len(moduli))
+    if rc:
+        raise ValueError(f"oracle_inverse64 failed rc={rc}")
+    return out
 
 
 def fill_random_limbs(logN: int, moduli, seed: int) -> np.ndarray:

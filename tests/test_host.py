@@ -152,3 +152,13 @@ def test_lds_limit_enforced():
     # a 2^15-word column tile is 132 KiB with padding (fits 160 KiB); 2^16 does not
     assert ntt_amd.validate_config(17, ntt_amd.TransformConfig.make((10, 5, 5), (7, 3, 0))) == 0
     assert ntt_amd.validate_config(17, ntt_amd.TransformConfig.make((10, 5, 6), (7, 3, 0))) != 0
+
+
+def test_generate_moduli64_matches_python_search():
+    """ntt_generate_moduli64 against a Python-integer search (Miller-Rabin,
+    Euler's criterion): same primes, same smallest quadratic non-residues."""
+    from tests.test_oracle import _primes64
+    got = [tuple(m) for m in ntt_amd.generate_moduli64(17, 4)]
+    assert got == _primes64(4, logN=17)
+    for q, g in ntt_amd.generate_moduli64(12, 3, 1 << 40):
+        assert q < (1 << 40) and (q - 1) % (1 << 13) == 0 and pow(g, (q - 1) // 2, q) == q - 1

@@ -301,3 +301,77 @@ def test_planted_mistake_twiddle_split_is_caught(oracle_mod):
     i = np.arange(1 << logN)
     assert np.array_equal(tw, tw[i & 63] * tw[(i >> 6) << 6] % q)
     assert not np.array_equal(tw, tw[i & 63] * tw[i >> 6] % q)
+
+
+# (q, a quadratic non-residue) with q < 2^62, q = 1 mod 2^18, found with
+# Python integers (Miller-Rabin over fixed bases, Euler's criterion)
+def _primes64(count, logN=17, bound=1 << 62):
+    def is_prime(n):
+        if n < 2:
+            return False
+        for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+            if n % p == 0:
+                return n == p
+        d, s = n - 1, 0
+        while d % 2 == 0:
+            d //= 2
+            s += 1
+        for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+            x = pow(a, d, n)
+            if x in (1, n - 1):
+                continue
+            for _ in range(s - 1):
+                x = x * x % n
+                if x == n - 1:
+                    break
+            else:
+                return False
+        return True
+    step = 1 << (logN + 1)
+    q = (bound - 1) // step * step + 1
+    out = []
+    while len(out) < count:
+        if q < bound and is_prime(q):
+            g = 2
+            while pow(g, (q - 1) // 2, q) != q - 1:
+                g += 1
+            out.append((q, g))
+        q -= step
+    return out
+
+
+def test_oracle64_equals_32bit_oracle_below_2_31(oracle_mod):
+    """For q < 2^31 the 64-bit transforms equal the 32-bit oracle's plain variant."""
+    logN = 11
+    moduli = [(998244353, 3), (1073479681, 11), (12289, 11)]
+    x = oracle_mod.fill_random_limbs(logN, moduli, 5)
+    f32 = oracle_mod.forward(x, logN, moduli, oracle_mod.PLAIN)
+    f64 = oracle_mod.forward64(x.astype(np.uint64), logN, moduli)
+    assert np.array_equal(f64, f32.astype(np.uint64))
+    assert np.array_equal(oracle_mod.inverse64(f64, logN, moduli), x.astype(np.uint64))
+
+
+def test_oracle64_brute_force_and_roundtrip(oracle_mod):
+    """62-bit primes: evaluations at psi^(2 brv(k) + 1) with Python integers
+    (N = 2^6), and forward/inverse roundtrip at N = 2^12."""
+    moduli = _primes64(2)
+    logN = 6
+    N = 1 << logN
+    rng = random.Random(3)
+    for q, g in moduli:
+        psi = oracle_mod.psi64(logN, q, g)
+        assert psi and pow(psi, N, q) == q - 1
+        a = [rng.randrange(q) for _ in range(N)]
+        got = oracle_mod.forward64(np.array([a], dtype=np.uint64), logN, [(q, g)])[0]
+        for k in range(N):
+            e = 2 * oracle_mod.bit_reverse(k, logN) + 1
+            assert int(got[k]) == sum(a[j] * pow(psi, e * j, q) for j in range(N)) % q
+    logN = 12
+    x = np.array([[rng.randrange(q) for _ in range(1 << logN)] for q, _ in moduli], dtype=np.uint64)
+    assert np.array_equal(oracle_mod.inverse64(oracle_mod.forward64(x, logN, moduli), logN, moduli), x)
+
+
+def test_planted_mistake_oracle64_non_primitive_root_is_refused(oracle_mod):
+    """A root whose psi is not primitive (a quadratic residue: 4) is refused."""
+    q, _ = _primes64(1)[0]
+    assert oracle_mod.psi64(10, q, 4) == 0
