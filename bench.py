@@ -86,6 +86,46 @@ def hbm_traffic(path: str, kernel: str, workload: str):
     return None
 
 
+def bench_word64(ntt_amd, dev, stream, logN: int, limbs: int, pool_bytes: int = 1 << 30, reps: int = 5):
+    """64-bit words (ntt_plan64): forward + inverse over a 1 GiB pool of
+    operands of `limbs` 62-bit limbs, against the live 64-bit butterfly roof."""
+    import numpy as np
+    moduli = ntt_amd.generate_moduli64(logN, limbs)
+    N = 1 << logN
+    count = max(1, pool_bytes // (8 * limbs * N))
+    rng = np.random.default_rng(0)
+    one = np.stack([rng.integers(0, int(m.q), N, dtype=np.uint64) for m in moduli])
+    pool = dev.DeviceBuffer(8 * limbs * N * count)
+    for k in range(count):
+        pool.upload(one, offset_bytes=8 * limbs * N * k)
+    ctx = ntt_amd.Context(0)
+    plan = ctx.plan64(logN, moduli)
+    e0, e1 = dev.Event(), dev.Event()
+    plan.forward(pool, count, stream)
+    plan.inverse(pool, count, stream)
+    stream.synchronize()
+    out = {}
+    for name, fn in (("forward", lambda: plan.forward(pool, count, stream)),
+                     ("inverse", lambda: plan.inverse(pool, count, stream))):
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        out[name + "_ms"] = round(e0.elapsed_ms(e1) / reps, 4)
+    # reps forwards then reps inverses: the pool must be back to its contents
+    roundtrip = bool(np.array_equal(pool.numpy(np.uint64, nbytes=8 * limbs * N).reshape(one.shape), one))
+    bfly = count * limbs * (N // 2) * logN
+    step_ms = out["forward_ms"] + out["inverse_ms"]
+    roof = dev.butterfly_roof(dev.BUTTERFLY_CT64, stream)
+    plan.close()
+    return {"workload": f"N=2^{logN}, {limbs} limbs of 62-bit primes, {count} operands ({8 * limbs * N * count >> 20} MiB)",
+            **out, "gbfly_s": round(2 * bfly / (step_ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(2 * 2 * 8 * limbs * N * count / (step_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "roof_ct64_gbfly_s": round(roof / 1e9, 1),
+            "forward_vs_roof": round(bfly / (out["forward_ms"] * 1e-3) / roof, 4), "roundtrip_ok": roundtrip}
+
+
 def bench_mixed(ntt_amd, dev, stream, reps: int = 10, tune: bool = True, seed: int = 0):
     """The batched uneven workload: 256 jobs, logN ~ U{12..17}, L ~ U{1..48},
     direction ~ Bernoulli(1/2), Montgomery domain, seed 0, HBM-resident.
@@ -490,6 +530,7 @@ def main():
                     ot[f"{tag}_{name}_ms"] = round(ev_a.elapsed_ms(ev_b) / 5, 4)
             plan.set_twiddle_mode(ntt_amd.TWIDDLE_TABLE)
             extras["twiddle_ot"] = ot
+        extras["word64"] = bench_word64(ntt_amd, dev, stream, logN, limbs)
         log("extras: " + json.dumps(extras))
 
     mixed = None

@@ -57,7 +57,8 @@ int ntt_rt_fill_uniform(uint32_t* dst, const uint32_t* moduli_dev, size_t limbs,
 /* Butterfly roof: butterflies per second the whole device sustains running
  * the transform kernels' own butterfly code on register-resident operands
  * (no memory or LDS traffic).  variant 0: Cooley-Tukey, canonical; 1: CT,
- * lazy (q < 2^30); 2: Gentleman-Sande, canonical; 3: GS, lazy.  Blocks until
+ * lazy (q < 2^30); 2: Gentleman-Sande, canonical; 3: GS, lazy; 4: CT and
+ * 5: GS on 64-bit words (q < 2^62, the ntt_plan64 butterflies).  Blocks until
  * done (about a millisecond). */
 int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream);
 /* Plain streaming copy of `bytes` (multiple of 16, 16-byte aligned): reads and

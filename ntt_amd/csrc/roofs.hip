@@ -11,6 +11,7 @@
 #include "../../include/ntt_amd.h"
 #include "../../include/ntt_amd_runtime.h"
 #include "arith.hpp"
+#include "arith64.hpp"
 #include "launch.hpp"
 
 namespace {
@@ -52,6 +53,32 @@ __global__ __launch_bounds__(ROOF_THREADS) void butterfly_roof_kernel(uint32_t* 
     out[t] = r;
 }
 
+// The same for 64-bit words (variant 4: Cooley-Tukey, 5: Gentleman-Sande).
+template <int VARIANT>
+__global__ __launch_bounds__(ROOF_THREADS) void butterfly_roof64_kernel(uint32_t* out, uint64_t q, uint64_t q_inv,
+                                                                        uint64_t w0) {
+    uint64_t X[ROOF_CHAINS], Y[ROOF_CHAINS];
+    const uint32_t t = blockIdx.x * ROOF_THREADS + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < ROOF_CHAINS; ++i) {
+        X[i] = ((uint64_t)t * 7u + (uint64_t)i) % q;
+        Y[i] = ((uint64_t)t * 13u + 5u * (uint64_t)i) % q;
+    }
+    uint64_t w = w0;
+    for (int it = 0; it < ROOF_ITERS / 4; ++it) {
+#pragma unroll
+        for (int i = 0; i < ROOF_CHAINS; ++i) {
+            if (VARIANT == 4) ntt::ct_butterfly64(X[i], Y[i], w, q, q_inv);
+            if (VARIANT == 5) ntt::gs_butterfly64(X[i], Y[i], w, q, q_inv);
+        }
+        w = w + 1u < q ? w + 1u : 1u;
+    }
+    uint64_t r = 0;
+#pragma unroll
+    for (int i = 0; i < ROOF_CHAINS; ++i) r ^= X[i] ^ Y[i];
+    out[t] = (uint32_t)(r ^ (r >> 32));
+}
+
 __global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
                                                           size_t n16) {
     const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -63,7 +90,7 @@ __global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restric
 extern "C" {
 
 int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream) {
-    if (!butterflies_per_s || variant < 0 || variant > 3) return NTT_ERR_INVALID_ARGUMENT;
+    if (!butterflies_per_s || variant < 0 || variant > 5) return NTT_ERR_INVALID_ARGUMENT;
     int dev = 0, cus = 0;
     int rc = ck(hipGetDevice(&dev), "hipGetDevice");
     if (rc) return rc;
@@ -83,16 +110,30 @@ int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream) 
     uint32_t inv = 1;
     for (int i = 0; i < 5; ++i) inv *= 2u - q * inv;
     const uint32_t q_inv = 0u - inv;
+    const uint64_t q64 = 4611686018425815041ull;   // the largest prime < 2^62 with q = 1 mod 2^18
+    uint64_t inv64 = 1;
+    for (int i = 0; i < 6; ++i) inv64 *= 2u - q64 * inv64;
+    const uint64_t q_inv64 = 0u - inv64;
+    const bool w64 = variant >= 4;
+    void (*fn64)(uint32_t*, uint64_t, uint64_t, uint64_t) =
+        variant == 4 ? butterfly_roof64_kernel<4> : butterfly_roof64_kernel<5>;
+    auto launch = [&](uint32_t salt) {
+        if (w64)
+            hipLaunchKernelGGL(fn64, dim3(blocks), dim3(ROOF_THREADS), 0, (hipStream_t)stream, out, q64, q_inv64,
+                               (uint64_t)salt * 0x9E3779B97F4A7C15ull % q64);
+        else
+            hipLaunchKernelGGL(fn, dim3(blocks), dim3(ROOF_THREADS), 0, (hipStream_t)stream, out, q, q_inv, salt);
+    };
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     rc = ck(hipEventCreate(&e0), "hipEventCreate");
     if (!rc) rc = ck(hipEventCreate(&e1), "hipEventCreate");
     const int reps = 5;
     if (!rc) {
-        hipLaunchKernelGGL(fn, dim3(blocks), dim3(ROOF_THREADS), 0, s, out, q, q_inv, 12345u);   // warm-up
+        launch(12345u);   // warm-up
         rc = ck(hipEventRecord(e0, s), "hipEventRecord");
         for (int r = 0; r < reps && !rc; ++r) {
-            hipLaunchKernelGGL(fn, dim3(blocks), dim3(ROOF_THREADS), 0, s, out, q, q_inv, 777u + (uint32_t)r);
+            launch(777u + (uint32_t)r);
             rc = ck(hipGetLastError(), "butterfly roof launch");
         }
         if (!rc) rc = ck(hipEventRecord(e1, s), "hipEventRecord");
@@ -100,7 +141,7 @@ int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream) 
         float ms = 0;
         if (!rc) rc = ck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
         if (!rc) {
-            const double bfly = (double)reps * blocks * ROOF_THREADS * ROOF_CHAINS * ROOF_ITERS;
+            const double bfly = (double)reps * blocks * ROOF_THREADS * ROOF_CHAINS * (w64 ? ROOF_ITERS / 4 : ROOF_ITERS);
             *butterflies_per_s = ms > 0 ? bfly / (ms * 1e-3) : 0.0;
         }
     }

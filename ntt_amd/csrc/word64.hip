@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/ntt_amd.h"
+#include "arith64.hpp"
 #include "context.hpp"
 #include "host_math.hpp"
 #include "launch.hpp"
@@ -24,23 +25,6 @@ namespace ntt {
 
 typedef __attribute__((address_space(1))) uint64_t gu64;
 typedef __attribute__((address_space(1))) const uint64_t gcu64;
-
-// a * b * 2^-64 mod q for a, b < q < 2^63, q_inv = -q^-1 mod 2^64:
-// t + m q = 2^64 (hi(t) + hi(m q) + carry) with carry = (lo(t) != 0), since
-// lo(t) + lo(m q) = 0 mod 2^64; the result lies in [0, 2q).
-__device__ __forceinline__ uint64_t mont_mul64(uint64_t a, uint64_t b, uint64_t q, uint64_t q_inv) {
-    const uint64_t tlo = a * b, thi = __umul64hi(a, b);
-    const uint64_t m = tlo * q_inv;
-    const uint64_t u = thi + __umul64hi(m, q) + (tlo != 0 ? 1u : 0u);
-    return u >= q ? u - q : u;
-}
-__device__ __forceinline__ uint64_t add_mod64(uint64_t a, uint64_t b, uint64_t q) {
-    const uint64_t s = a + b;   // < 2q < 2^63
-    return s >= q ? s - q : s;
-}
-__device__ __forceinline__ uint64_t sub_mod64(uint64_t a, uint64_t b, uint64_t q) {
-    return a >= b ? a - b : a + (q - b);
-}
 
 struct Pass64Params {
     uint64_t* A;                       // operand k, limb l at A + (k * limbs + l) * N
@@ -95,15 +79,13 @@ __global__ __launch_bounds__(PASS64_THREADS) void pass64_kernel(Pass64Params p) 
             }
             const uint32_t e1 = e0 + (COL ? (1u << (bl + W)) : (1u << bl));
             const uint64_t w = tw[(1u << (n - 1 - bg)) + hi];
-            const uint64_t x = L64[e0], y = L64[e1];
-            if (INV) {
-                L64[e0] = add_mod64(x, y, q);
-                L64[e1] = mont_mul64(sub_mod64(x, y, q), w, q, q_inv);
-            } else {
-                const uint64_t t = mont_mul64(y, w, q, q_inv);
-                L64[e0] = add_mod64(x, t, q);
-                L64[e1] = sub_mod64(x, t, q);
-            }
+            uint64_t x = L64[e0], y = L64[e1];
+            if (INV)
+                gs_butterfly64(x, y, w, q, q_inv);
+            else
+                ct_butterfly64(x, y, w, q, q_inv);
+            L64[e0] = x;
+            L64[e1] = y;
         }
         __syncthreads();
     }
