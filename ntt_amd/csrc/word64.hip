@@ -8,8 +8,8 @@
 // plain product y * psi^e mod q, and the inverse ends with mont_mul(x,
 // to_mont(N^-1)).  Two passes like the 32-bit two-pass engine: a column pass
 // over the top R1 index bits and a row pass over the low R2 bits; each pass
-// stages a tile in LDS and runs its stages as radix-2 butterflies from LDS,
-// one barrier per stage (the 64-bit product chain, not LDS traffic, bounds it).
+// stages a tile in LDS and runs its stages from LDS two at a time (radix 4),
+// one barrier per two stages (the 64-bit product chain, not LDS traffic, bounds it).
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -59,33 +59,69 @@ __global__ __launch_bounds__(PASS64_THREADS) void pass64_kernel(Pass64Params p) 
     };
     for (uint32_t e = threadIdx.x; e < words; e += PASS64_THREADS) L64[e] = A[gidx(e)];
     __syncthreads();
-    for (int s = 0; s < R; ++s) {
-        const int bl = INV ? s : R - 1 - s;            // pass-local bit of this stage
-        const int bg = COL ? n - R + bl : bl;          // bit of the word index
-        const uint32_t lo_mask = (1u << bl) - 1;
-        for (uint32_t pi = threadIdx.x; pi < words / 2; pi += PASS64_THREADS) {
-            uint32_t e0;
-            uint32_t hi;   // word index >> (bg + 1) of the pair's first word
-            if (COL) {
-                const uint32_t c = pi & mask, rp = pi >> W;
-                const uint32_t r0 = ((rp >> bl) << (bl + 1)) | (rp & lo_mask);
-                e0 = (r0 << W) | c;
-                hi = r0 >> (bl + 1);
-            } else {
-                const uint32_t rib = pi >> (R - 1), vp = pi & ((1u << (R - 1)) - 1);
-                const uint32_t v0 = ((vp >> bl) << (bl + 1)) | (vp & lo_mask);
-                e0 = (rib << R) | v0;
-                hi = ((((tile << W) + rib) << R) + v0) >> (bg + 1);
+    // Stages two at a time (radix 4: a quad of words over bits lb and lb + 1,
+    // one LDS round trip and one barrier per two stages); a lone last stage
+    // when R is odd.  Forward: bits from the top down; inverse: from the bottom up.
+    auto elem = [&](uint32_t r_or_v, uint32_t lane_part) -> uint32_t {   // tile element of a pass index
+        return COL ? (r_or_v << W) | lane_part : (lane_part << R) | r_or_v;
+    };
+    // word index >> (bit + 1) of pass index x (x: row for COL, position in the row otherwise)
+    auto above = [&](uint32_t x, uint32_t lane_part, int bl) -> uint32_t {
+        return COL ? x >> (bl + 1) : ((((tile << W) + lane_part) << R) + x) >> (bl + 1);
+    };
+    auto tw_at = [&](int bl, uint32_t a) -> uint64_t {
+        const int bg = COL ? n - R + bl : bl;
+        return tw[(1u << (n - 1 - bg)) + a];
+    };
+    for (int s = 0; s < R;) {
+        if (R - s >= 2) {
+            const int lb = INV ? s : R - 2 - s, hb = lb + 1;
+            const uint32_t lo_mask = (1u << lb) - 1;
+            for (uint32_t qi = threadIdx.x; qi < words / 4; qi += PASS64_THREADS) {
+                const uint32_t lp = COL ? (qi & mask) : qi >> (R - 2);
+                const uint32_t xp = COL ? qi >> W : qi & ((1u << (R - 2)) - 1);
+                const uint32_t x0 = ((xp >> lb) << (lb + 2)) | (xp & lo_mask);
+                const uint32_t e0 = elem(x0, lp), e1 = elem(x0 + (1u << lb), lp);
+                const uint32_t e2 = elem(x0 + (1u << hb), lp), e3 = elem(x0 + (1u << hb) + (1u << lb), lp);
+                const uint64_t wh = tw_at(hb, above(x0, lp, hb));
+                const uint32_t al = above(x0, lp, lb);
+                const uint64_t wl0 = tw_at(lb, al), wl1 = tw_at(lb, al + 1);
+                uint64_t a0 = L64[e0], a1 = L64[e1], a2 = L64[e2], a3 = L64[e3];
+                if (INV) {
+                    gs_butterfly64(a0, a1, wl0, q, q_inv);
+                    gs_butterfly64(a2, a3, wl1, q, q_inv);
+                    gs_butterfly64(a0, a2, wh, q, q_inv);
+                    gs_butterfly64(a1, a3, wh, q, q_inv);
+                } else {
+                    ct_butterfly64(a0, a2, wh, q, q_inv);
+                    ct_butterfly64(a1, a3, wh, q, q_inv);
+                    ct_butterfly64(a0, a1, wl0, q, q_inv);
+                    ct_butterfly64(a2, a3, wl1, q, q_inv);
+                }
+                L64[e0] = a0;
+                L64[e1] = a1;
+                L64[e2] = a2;
+                L64[e3] = a3;
             }
-            const uint32_t e1 = e0 + (COL ? (1u << (bl + W)) : (1u << bl));
-            const uint64_t w = tw[(1u << (n - 1 - bg)) + hi];
-            uint64_t x = L64[e0], y = L64[e1];
-            if (INV)
-                gs_butterfly64(x, y, w, q, q_inv);
-            else
-                ct_butterfly64(x, y, w, q, q_inv);
-            L64[e0] = x;
-            L64[e1] = y;
+            s += 2;
+        } else {
+            const int bl = INV ? s : R - 1 - s;
+            const uint32_t lo_mask = (1u << bl) - 1;
+            for (uint32_t pi = threadIdx.x; pi < words / 2; pi += PASS64_THREADS) {
+                const uint32_t lp = COL ? (pi & mask) : pi >> (R - 1);
+                const uint32_t xp = COL ? pi >> W : pi & ((1u << (R - 1)) - 1);
+                const uint32_t x0 = ((xp >> bl) << (bl + 1)) | (xp & lo_mask);
+                const uint32_t e0 = elem(x0, lp), e1 = elem(x0 + (1u << bl), lp);
+                const uint64_t w = tw_at(bl, above(x0, lp, bl));
+                uint64_t x = L64[e0], y = L64[e1];
+                if (INV)
+                    gs_butterfly64(x, y, w, q, q_inv);
+                else
+                    ct_butterfly64(x, y, w, q, q_inv);
+                L64[e0] = x;
+                L64[e1] = y;
+            }
+            s += 1;
         }
         __syncthreads();
     }
