@@ -167,9 +167,10 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
         a.final_pass = c2.radix_stages == 0;
         b.final_pass = 1;
         (first_is_col ? a : b).src = L.src;
-        PhaseParams& fin = c2.radix_stages > 0 ? b : a;
-        fin.sub = L.sub;
-        fin.post = L.post;
+        // the mod-down store is in the row pass, where the forward transform ends
+        if (L.sub && c2.radix_stages == 0) return NTT_ERR_CONFIG;
+        b.sub = L.sub;
+        b.post = L.post;
         if ((pass_mask & 1) && (rc = launch_pass(c1, false, true, mode, a, stream))) return rc;
         if ((pass_mask & 2) && (rc = launch_pass(c2, false, false, mode, b, stream))) return rc;
     } else {

@@ -256,8 +256,9 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
 #pragma unroll
                 for (int x = 0; x < E; ++x) v[x] = canon_4q(v[x], q);
             }
-            if (!INV && p.sub) {
+            if (!INV && !COL && p.sub) {
                 // mod-down: (sub - NTT) * post, read at the positions stored below
+                // (row passes only: the forward transform ends in its row pass)
                 const uint32_t f = p.post[li];
                 uint32_t s[E];
                 gload((gcu32*)(p.sub + off), lo, s);
