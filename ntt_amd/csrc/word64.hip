@@ -241,7 +241,8 @@ int ntt_generate_moduli64(int logN, uint64_t bound, int count, ntt_modulus_confi
 
 int ntt_plan64_create(ntt_context* ctx, int logN, const ntt_modulus_config64* moduli, size_t limbs,
                       ntt_plan64** out) {
-    if (!ctx || !moduli || !out || limbs == 0 || logN < 2 || logN > 17) return NTT_ERR_INVALID_ARGUMENT;
+    if (!ctx || !moduli || !out || limbs == 0 || limbs > 0xffffffu || logN < 2 || logN > 17)
+        return NTT_ERR_INVALID_ARGUMENT;
     const size_t N = (size_t)1 << logN;
     std::vector<uint64_t> tf(limbs * N), ti(limbs * N), q(limbs), qi(limbs), ninv(limbs);
     for (size_t l = 0; l < limbs; ++l) {
