@@ -8,8 +8,8 @@
 // plain product y * psi^e mod q, and the inverse ends with mont_mul(x,
 // to_mont(N^-1)).  Two passes like the 32-bit two-pass engine: a column pass
 // over the top R1 index bits and a row pass over the low R2 bits; each pass
-// stages a tile in LDS and runs its stages from LDS two at a time (radix 4),
-// one barrier per two stages (the 64-bit product chain, not LDS traffic, bounds it).
+// stages a tile in LDS and runs its stages three at a time in registers
+// (groups of 8 words), one barrier per three stages.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -38,6 +38,56 @@ struct Pass64Params {
 
 constexpr int PASS64_THREADS = 256;
 
+namespace w64 {
+
+// One round of K stages on pass bits [lb, lb + K) over a tile in LDS.  A
+// thread takes groups of 2^K words whose pass indices differ only in those
+// bits (group member m at x0 + (m << lb)), runs the K stages in registers and
+// writes the group back.  For a word at pass index x, (word index) >> (b + 1)
+// is x >> (b + 1) in a column pass and ((row << R) + x) >> (b + 1) in a row
+// pass; for group members it is that of x0 plus m >> (kb + 1) (x0 is zero on
+// the group bits, so nothing carries).
+template <int K, bool INV, bool COL>
+__device__ __forceinline__ void group_round(uint64_t* L64, gcu64* tw, uint64_t q, uint64_t q_inv, int n, int R, int W,
+                                            uint32_t tile, int lb) {
+    constexpr int G = 1 << K;
+    const uint32_t words = 1u << (R + W), mask = (1u << W) - 1, lo_mask = (1u << lb) - 1;
+    for (uint32_t gi = threadIdx.x; gi < words / G; gi += PASS64_THREADS) {
+        const uint32_t lp = COL ? (gi & mask) : gi >> (R - K);
+        const uint32_t xp = COL ? gi >> W : gi & ((1u << (R - K)) - 1);
+        const uint32_t x0 = ((xp >> lb) << (lb + K)) | (xp & lo_mask);
+        const uint32_t base = COL ? x0 : (((tile << W) + lp) << R) + x0;   // (word index) above, before shifting
+        uint32_t e[G];
+        uint64_t a[G];
+#pragma unroll
+        for (int m = 0; m < G; ++m) {
+            const uint32_t x = x0 + ((uint32_t)m << lb);
+            e[m] = COL ? (x << W) | lp : (lp << R) | x;
+            a[m] = L64[e[m]];
+        }
+#pragma unroll
+        for (int st = 0; st < K; ++st) {
+            const int kb = INV ? st : K - 1 - st;          // group bit of this stage
+            const int b = lb + kb;                         // pass bit
+            const int bg = COL ? n - R + b : b;            // word-index bit
+            const uint32_t tbase = (1u << (n - 1 - bg)) + (base >> (b + 1));
+#pragma unroll
+            for (int m = 0; m < G; ++m) {
+                if (m & (1 << kb)) continue;
+                const uint64_t w = tw[tbase + ((uint32_t)m >> (kb + 1))];
+                if (INV)
+                    gs_butterfly64(a[m], a[m | (1 << kb)], w, q, q_inv);
+                else
+                    ct_butterfly64(a[m], a[m | (1 << kb)], w, q, q_inv);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < G; ++m) L64[e[m]] = a[m];
+    }
+}
+
+}  // namespace w64
+
 // COL: the top R bits of the index, tiles of 2^W consecutive columns of the
 // 2^R x 2^(n-R) matrix; row pass: the low R bits, tiles of 2^W rows.
 template <bool INV, bool COL>
@@ -59,70 +109,20 @@ __global__ __launch_bounds__(PASS64_THREADS) void pass64_kernel(Pass64Params p) 
     };
     for (uint32_t e = threadIdx.x; e < words; e += PASS64_THREADS) L64[e] = A[gidx(e)];
     __syncthreads();
-    // Stages two at a time (radix 4: a quad of words over bits lb and lb + 1,
-    // one LDS round trip and one barrier per two stages); a lone last stage
-    // when R is odd.  Forward: bits from the top down; inverse: from the bottom up.
-    auto elem = [&](uint32_t r_or_v, uint32_t lane_part) -> uint32_t {   // tile element of a pass index
-        return COL ? (r_or_v << W) | lane_part : (lane_part << R) | r_or_v;
-    };
-    // word index >> (bit + 1) of pass index x (x: row for COL, position in the row otherwise)
-    auto above = [&](uint32_t x, uint32_t lane_part, int bl) -> uint32_t {
-        return COL ? x >> (bl + 1) : ((((tile << W) + lane_part) << R) + x) >> (bl + 1);
-    };
-    auto tw_at = [&](int bl, uint32_t a) -> uint64_t {
-        const int bg = COL ? n - R + bl : bl;
-        return tw[(1u << (n - 1 - bg)) + a];
-    };
+    // Stages K = 3 at a time (a group of 8 words over pass bits lb .. lb + 2:
+    // one LDS round trip and one barrier per three stages, 12 butterflies and 7
+    // twiddle loads per group); a last round of 1 or 2 stages when 3 does not
+    // divide R.  Forward: bits from the top down; inverse: from the bottom up.
     for (int s = 0; s < R;) {
-        if (R - s >= 2) {
-            const int lb = INV ? s : R - 2 - s, hb = lb + 1;
-            const uint32_t lo_mask = (1u << lb) - 1;
-            for (uint32_t qi = threadIdx.x; qi < words / 4; qi += PASS64_THREADS) {
-                const uint32_t lp = COL ? (qi & mask) : qi >> (R - 2);
-                const uint32_t xp = COL ? qi >> W : qi & ((1u << (R - 2)) - 1);
-                const uint32_t x0 = ((xp >> lb) << (lb + 2)) | (xp & lo_mask);
-                const uint32_t e0 = elem(x0, lp), e1 = elem(x0 + (1u << lb), lp);
-                const uint32_t e2 = elem(x0 + (1u << hb), lp), e3 = elem(x0 + (1u << hb) + (1u << lb), lp);
-                const uint64_t wh = tw_at(hb, above(x0, lp, hb));
-                const uint32_t al = above(x0, lp, lb);
-                const uint64_t wl0 = tw_at(lb, al), wl1 = tw_at(lb, al + 1);
-                uint64_t a0 = L64[e0], a1 = L64[e1], a2 = L64[e2], a3 = L64[e3];
-                if (INV) {
-                    gs_butterfly64(a0, a1, wl0, q, q_inv);
-                    gs_butterfly64(a2, a3, wl1, q, q_inv);
-                    gs_butterfly64(a0, a2, wh, q, q_inv);
-                    gs_butterfly64(a1, a3, wh, q, q_inv);
-                } else {
-                    ct_butterfly64(a0, a2, wh, q, q_inv);
-                    ct_butterfly64(a1, a3, wh, q, q_inv);
-                    ct_butterfly64(a0, a1, wl0, q, q_inv);
-                    ct_butterfly64(a2, a3, wl1, q, q_inv);
-                }
-                L64[e0] = a0;
-                L64[e1] = a1;
-                L64[e2] = a2;
-                L64[e3] = a3;
-            }
-            s += 2;
-        } else {
-            const int bl = INV ? s : R - 1 - s;
-            const uint32_t lo_mask = (1u << bl) - 1;
-            for (uint32_t pi = threadIdx.x; pi < words / 2; pi += PASS64_THREADS) {
-                const uint32_t lp = COL ? (pi & mask) : pi >> (R - 1);
-                const uint32_t xp = COL ? pi >> W : pi & ((1u << (R - 1)) - 1);
-                const uint32_t x0 = ((xp >> bl) << (bl + 1)) | (xp & lo_mask);
-                const uint32_t e0 = elem(x0, lp), e1 = elem(x0 + (1u << bl), lp);
-                const uint64_t w = tw_at(bl, above(x0, lp, bl));
-                uint64_t x = L64[e0], y = L64[e1];
-                if (INV)
-                    gs_butterfly64(x, y, w, q, q_inv);
-                else
-                    ct_butterfly64(x, y, w, q, q_inv);
-                L64[e0] = x;
-                L64[e1] = y;
-            }
-            s += 1;
-        }
+        const int K = R - s >= 3 ? 3 : R - s;
+        const int lb = INV ? s : R - s - K;
+        if (K == 3)
+            w64::group_round<3, INV, COL>(L64, tw, q, q_inv, n, R, W, tile, lb);
+        else if (K == 2)
+            w64::group_round<2, INV, COL>(L64, tw, q, q_inv, n, R, W, tile, lb);
+        else
+            w64::group_round<1, INV, COL>(L64, tw, q, q_inv, n, R, W, tile, lb);
+        s += K;
         __syncthreads();
     }
     const uint64_t sc = p.scale ? p.scale[li] : 0;
