@@ -32,6 +32,10 @@
  * draw (ACM TOMACS 29(1), 2019), which is the algorithm GCC 11's libstdc++
  * selects for a 32-bit generator (bits/uniform_int_dist.h, _S_nd).
  *
+ * plus, at 64-bit word width (the last section of this file), the same
+ * textbook transforms for primes q < 2^62 (Cheddar's NTTHandler<uint64_t>
+ * word width, src/NTT_cheddar.cu:1313).
+ *
  * Pinning (each function is checked in tests/test_oracle.py):
  *   pow_mod, bit_reverse      Python pow(), known answers
  *   modulus runtime           psi^2N = 1, psi^N = q-1, N*N^-1 = 1, q*q_inv = -1
