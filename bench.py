@@ -305,7 +305,7 @@ def cpu_baseline(logN: int, moduli, seconds: float, threads: int = 1):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {args.gpus}). "
+        print(f"bench.py measures one GPU: the library runs on a single MI355X (asked for --gpus {args.gpus}). "
               "Limbs and batch operands shard across GPUs with no exchange.", file=sys.stderr)
         sys.exit(2)
 
