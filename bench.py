@@ -394,8 +394,8 @@ def main():
     ebi, _ = plan.fused_variant(B, ntt_amd.INVERSE)
     lazy = all(q < (1 << 30) for q, _ in moduli)   # two-pass kernels then use lazy reduction
     if engine_used == "fused":
-        kern = {f"fused_kernel<{logN}, false, {'true' if lzf else 'false'}, {ebf}, false>": lambda: plan.forward(pool, B, stream),
-                f"fused_kernel<{logN}, true, false, {ebi}, false>": lambda: plan.inverse(pool, B, stream)}
+        kern = {f"fused_kernel<{logN}, false, {'true' if lzf else 'false'}, {ebf}, false, false>": lambda: plan.forward(pool, B, stream),
+                f"fused_kernel<{logN}, true, false, {ebi}, false, false>": lambda: plan.inverse(pool, B, stream)}
     else:
         mode = 2 if lazy else 0
         c = cfg.astuple()
