@@ -2,8 +2,9 @@
 (ntt_plan_forward_to / ntt_plan_inverse_to / ntt_plan_forward_mod_down),
 bit-exact against the oracle (integer arithmetic: tolerance 0) on both
 engines, both domains, lazy-capable and canonical-only moduli, several
-operands per call, plus the exact-division identity at full size and the
-argument checks."""
+operands per call, plus the exact-division identity at full size, the
+argument checks, and hipGraph replays of a mod-down and an on-the-fly-twiddle
+launch."""
 import numpy as np
 import pytest
 
@@ -179,3 +180,43 @@ def test_mod_down_argument_errors(env):
     plan.set_mod_down_factors(None)
     with pytest.raises(ntt_amd.NTTError):
         plan.forward_mod_down(a, b, b)              # factors released
+
+
+def test_mod_down_and_ot_graph_replays(env):
+    """A mod-down launch and an on-the-fly-twiddle forward captured in one
+    hipGraph; three replays with eager work and synchronizations between
+    them, each checked against the oracle."""
+    ntt_amd, gu, orc = env
+    logN, limbs = 14, 4
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    x = gu.random_operand(logN, moduli, 71)
+    sub = gu.random_operand(logN, moduli, 72)
+    f = [5, 7, 11, 13]
+    ctx = ntt_amd.Context(0)
+    plan = ctx.plan(logN, moduli, 1)
+    plan.set_engine(ntt_amd.ENGINE_FUSED)
+    plan.set_mod_down_factors(f)
+    ot = ctx.plan(logN, moduli, 1)
+    ot.set_engine(ntt_amd.ENGINE_FUSED)
+    ot.set_twiddle_mode(ntt_amd.TWIDDLE_OT)
+    src, sb = gu.to_dev(x), gu.to_dev(sub)
+    dst, fw = ntt_amd.DeviceBuffer(x.nbytes), ntt_amd.DeviceBuffer(x.nbytes)
+    s = ntt_amd.Stream()
+    with ntt_amd.device.Graph.capture(s) as g:
+        plan.forward_mod_down(dst, src, sb, 1, s)
+        fw.copy_from(src, stream=s)
+        ot.forward(fw, 1, s)
+    exp_md = orc.mod_down(orc.forward(x, logN, moduli, 1), sub, f, moduli)
+    exp_fw = orc.forward(x, logN, moduli, 1)
+    for r in range(3):
+        dst.zero()
+        fw.zero()
+        gu.sync()
+        _print("case graph mod-down/ot", r)
+        g.launch(s)
+        s.synchronize()
+        assert np.array_equal(gu.to_host(dst).reshape(x.shape), exp_md), r
+        assert np.array_equal(gu.to_host(fw).reshape(x.shape), exp_fw), r
+        plan.forward(fw, 1, s)          # eager work between replays
+        s.synchronize()
+    g.close()
