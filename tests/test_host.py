@@ -162,3 +162,14 @@ def test_generate_moduli64_matches_python_search():
     assert got == _primes64(4, logN=17)
     for q, g in ntt_amd.generate_moduli64(12, 3, 1 << 40):
         assert q < (1 << 40) and (q - 1) % (1 << 13) == 0 and pow(g, (q - 1) // 2, q) == q - 1
+
+
+def test_generate_moduli64_arguments():
+    with pytest.raises(ntt_amd.NTTError):
+        ntt_amd.generate_moduli64(0, 2)
+    with pytest.raises(ntt_amd.NTTError):
+        ntt_amd.generate_moduli64(30, 2)
+    assert ntt_amd.generate_moduli64(12, 0) == []
+    # a bound too small for any q = 1 mod 2N yields none; the bound is capped at 2^62
+    assert ntt_amd.generate_moduli64(12, 3, 1 << 13) == []
+    assert all(m.q < (1 << 62) for m in ntt_amd.generate_moduli64(12, 2, 1 << 63))
