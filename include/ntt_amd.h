@@ -102,7 +102,9 @@ const char* ntt_status_string(int status);
 /* Text of the last HIP error this thread's calls turned into NTT_ERR_HIP. */
 const char* ntt_last_hip_error(void);
 const char* ntt_version(void);
-/* Max logN supported by the kernels (17 for the default phase split, up to 20). */
+/* Max logN supported (20).  The fused engine covers 10..16; the two-pass
+ * engine every size, with the reference's phase split up to logN 18 and the
+ * wave64 default split (ntt_default_transform_config) up to 20. */
 int ntt_max_log_n(void);
 
 /* ---------------------------------------------------------------------------
@@ -174,8 +176,9 @@ int ntt_launch_intt_plain(uint32_t* A_dev, const uint32_t* inv_twiddle_dev,
                           void* stream);
 
 /* ---------------------------------------------------------------------------
- * Context + plans: device-resident Montgomery twiddle cache keyed by (q, logN),
- * replacing the per-call table setup in the reference's main()
+ * Context + plans: device-resident Montgomery twiddle cache keyed by
+ * (q, 2N-th root, logN, direction) (two primitive roots of one q give two
+ * tables), replacing the per-call table setup in the reference's main()
  * (src/v04_montgomery.cu:882-916, 947-967).
  * ------------------------------------------------------------------------- */
 typedef struct ntt_context ntt_context;
@@ -432,7 +435,10 @@ int ntt_server_state(ntt_server* server, uint64_t id, int* state);
 /* Blocks until request `id` is done (or failed: its status is returned). */
 int ntt_server_wait(ntt_server* server, uint64_t id);
 /* Copies a finished request's result (limbs x 2^logN words) to `dst` (device
- * memory if dst_is_device, else host) and, if release, frees its operand. */
+ * memory if dst_is_device, else host) and, if release, frees its operand.
+ * The result stays allocated while any fetch of it is copying: a release (or
+ * a fetch with release) that arrives meanwhile takes effect when the last
+ * such copy ends.  After a release, fetch returns NTT_ERR_NOT_FOUND. */
 int ntt_server_fetch(ntt_server* server, uint64_t id, uint32_t* dst, int dst_is_device, int release);
 int ntt_server_release(ntt_server* server, uint64_t id);
 int ntt_server_stats_get(ntt_server* server, ntt_server_stats* out);

@@ -561,6 +561,12 @@ class Server:
             self._keep.pop(rid, None)
         return out
 
+    def fetch_device(self, rid: int, dst, release: bool = True):
+        """Copies a finished result into device buffer `dst`."""
+        _check(_lib.ntt_server_fetch(self.handle, rid, _addr(dst), 1, int(release)), "ntt_server_fetch")
+        if release:
+            self._keep.pop(rid, None)
+
     def release(self, rid: int):
         _check(_lib.ntt_server_release(self.handle, rid), "ntt_server_release")
         self._keep.pop(rid, None)

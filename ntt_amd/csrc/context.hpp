@@ -14,13 +14,18 @@
 
 namespace ntt {
 
-// Montgomery-form bit-reversed power tables, one per (q, logN, direction),
-// resident in HBM for the life of the context.
+// Montgomery-form bit-reversed power tables, one per (q, root, logN,
+// direction), resident in HBM for the life of the context.  The table is a
+// function of the 2N-th root psi (forward) or psi^-1 (inverse), not of q
+// alone: two primitive roots g of one q give different psi = g^((q-1)/2N)
+// (src/v04_montgomery.cu:681-727), so the root is part of the key.
 struct TwiddleCache {
-    std::mutex mu;
-    std::map<std::tuple<uint32_t, int, int>, uint32_t*> tables;
+    mutable std::mutex mu;
+    std::map<std::tuple<uint32_t, uint32_t, int, int>, uint32_t*> tables;
     size_t bytes = 0;
     int get(uint32_t q, uint32_t root, int logN, bool inverse, const uint32_t** out);
+    size_t entries() const;
+    size_t total_bytes() const;
     ~TwiddleCache();
 };
 

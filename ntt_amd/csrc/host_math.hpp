@@ -6,7 +6,9 @@
 
 #include "../../include/ntt_amd.h"
 
-#define NTT_MAX_LOG_N 24
+// Largest transform size accepted (the two-pass engine; every size up to it
+// is checked against the oracle on the GPU).
+#define NTT_MAX_LOG_N 20
 #define NTT_MAX_STAGE_MERGING 5
 // Sizes the fused limb kernel handles (one workgroup holds 2^16 words).
 #define FUSED_MIN_LOG_N 10

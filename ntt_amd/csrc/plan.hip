@@ -2,7 +2,7 @@
 //
 // The reference builds every table on the host and uploads it inside main()
 // for one fixed modulus set (src/v04_montgomery.cu:882-916, 947-967).  Here a
-// context owns one Montgomery-form table per (q, logN, direction), shared by
+// context owns one Montgomery-form table per (q, root, logN, direction), shared by
 // every plan and batch that uses that modulus, and kept in HBM until the
 // context is destroyed.  A plan binds a modulus list to device arrays of
 // per-limb constants and table pointers.
@@ -25,7 +25,7 @@ namespace ntt {
 
 int TwiddleCache::get(uint32_t q, uint32_t psi_root, int logN, bool inverse, const uint32_t** out) {
     std::lock_guard<std::mutex> g(mu);
-    auto key = std::make_tuple(q, logN, inverse ? 1 : 0);
+    auto key = std::make_tuple(q, psi_root, logN, inverse ? 1 : 0);
     auto it = tables.find(key);
     if (it != tables.end()) {
         *out = it->second;
@@ -45,6 +45,16 @@ int TwiddleCache::get(uint32_t q, uint32_t psi_root, int logN, bool inverse, con
     bytes += (N + M) * sizeof(uint32_t);
     *out = d;
     return NTT_OK;
+}
+
+size_t TwiddleCache::entries() const {
+    std::lock_guard<std::mutex> g(mu);
+    return tables.size();
+}
+
+size_t TwiddleCache::total_bytes() const {
+    std::lock_guard<std::mutex> g(mu);
+    return bytes;
 }
 
 TwiddleCache::~TwiddleCache() {
@@ -229,8 +239,8 @@ int ntt_context_destroy(ntt_context* ctx) {
     return NTT_OK;
 }
 
-size_t ntt_context_twiddle_bytes(const ntt_context* ctx) { return ctx ? ctx->twiddles.bytes : 0; }
-size_t ntt_context_twiddle_entries(const ntt_context* ctx) { return ctx ? ctx->twiddles.tables.size() : 0; }
+size_t ntt_context_twiddle_bytes(const ntt_context* ctx) { return ctx ? ctx->twiddles.total_bytes() : 0; }
+size_t ntt_context_twiddle_entries(const ntt_context* ctx) { return ctx ? ctx->twiddles.entries() : 0; }
 
 int ntt_plan_create(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs, int domain,
                     const ntt_transform_config* cfg, ntt_plan** out) {

@@ -49,6 +49,8 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr int SERVER_MAX_SLOTS = 8;
+// Poll interval of the scheduler while launches are in flight.
+constexpr int SERVER_POLL_US = 20;
 constexpr size_t ARENA_GRANULE = 256;
 
 int ck(hipError_t e, const char* where) { return e == hipSuccess ? NTT_OK : record_hip_error(e, where); }
@@ -107,6 +109,8 @@ struct Req {
     size_t bytes = 0;
     size_t off = 0;
     bool placed = false;            // holds arena space
+    int pins = 0;                   // fetches copying out of the arena right now
+    bool release_pending = false;   // released while pinned: freed by the last fetch
     int state = NTT_REQ_QUEUED;
     int status = NTT_OK;
     Clock::time_point arrival, done;
@@ -142,13 +146,18 @@ struct ntt_server {
     Slot slots[SERVER_MAX_SLOTS];
     int nslots = 0;
 
+    // Plans by request shape.  Guarded by plan_mu alone, so that building a
+    // new shape's tables (host work plus a blocking upload) never holds up
+    // the scheduler; plans are only added, and live until the server ends.
+    std::mutex plan_mu;
+    std::map<PlanKey, ntt_plan*> plans;
+    std::map<std::pair<int, uint32_t>, std::vector<ntt_modulus_config>> default_moduli;
+
     std::mutex mu;                  // guards everything below
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<Req>> queued;
     std::deque<std::shared_ptr<Req>> admitted;
     std::unordered_map<uint64_t, std::shared_ptr<Req>> live;
-    std::map<PlanKey, ntt_plan*> plans;
-    std::map<std::pair<int, uint32_t>, std::vector<ntt_modulus_config>> default_moduli;
     uint64_t next_id = 1;
     ntt_server_stats stats{};
     uint64_t jobs_launched = 0;
@@ -174,8 +183,9 @@ struct ntt_server {
 
 namespace {
 
-// Plan for a request's shape (caller holds mu).
+// Plan for a request's shape (takes plan_mu; the caller must not hold mu).
 int plan_for(ntt_server* S, const ntt_request& r, ntt_plan** out) {
+    std::lock_guard<std::mutex> g(S->plan_mu);
     const ntt_modulus_config* m = r.moduli;
     if (!m) {
         auto key = std::make_pair((int)r.logN, r.limbs);
@@ -445,14 +455,11 @@ void scheduler(ntt_server* S) {
         for (int i = 0; i < S->nslots; ++i) any_busy = any_busy || S->slots[i].busy;
         if (S->stop && !any_busy && S->admitted.empty() && S->queued.empty()) break;
         if (!progress) {
-            // in-flight work: poll again shortly; idle: sleep until a submit
-            if (any_busy) {
-                lk.unlock();
-                std::this_thread::yield();
-                lk.lock();
-            } else {
-                S->cv_work.wait_for(lk, std::chrono::milliseconds(5));
-            }
+            // In-flight work: poll the slots' events again after a short
+            // timed wait (a submit or release wakes it earlier); idle: sleep
+            // until a submit.
+            S->cv_work.wait_for(lk, any_busy ? std::chrono::microseconds(SERVER_POLL_US)
+                                             : std::chrono::microseconds(5000));
         }
     }
 }
@@ -536,9 +543,9 @@ int ntt_server_submit(ntt_server* server, const ntt_request* req, uint64_t* id) 
     q->bytes = ((size_t)r.limbs << r.logN) * sizeof(uint32_t);
     q->arrival = Clock::now();
     if (q->bytes > server->alloc.size()) return NTT_ERR_BUDGET;
-    std::lock_guard<std::mutex> g(server->mu);
     int rc = plan_for(server, r, &q->plan);
     if (rc) return rc;
+    std::lock_guard<std::mutex> g(server->mu);
     q->id = server->next_id++;
     server->live[q->id] = q;
     server->queued.push_back(q);
@@ -566,6 +573,19 @@ int ntt_server_wait(ntt_server* server, uint64_t id) {
     return q->status;
 }
 
+namespace {
+// Frees a finished request's arena range and forgets it (caller holds mu).
+void release_locked(ntt_server* S, const std::shared_ptr<Req>& q) {
+    if (q->placed) {
+        S->alloc.release(q->off, q->bytes);
+        q->placed = false;
+        S->stats.arena_bytes = S->alloc.used();
+    }
+    S->live.erase(q->id);
+    S->cv_work.notify_one();
+}
+}  // namespace
+
 int ntt_server_fetch(ntt_server* server, uint64_t id, uint32_t* dst, int dst_is_device, int release) {
     if (!server || !dst) return NTT_ERR_INVALID_ARGUMENT;
     std::shared_ptr<Req> q;
@@ -575,13 +595,17 @@ int ntt_server_fetch(ntt_server* server, uint64_t id, uint32_t* dst, int dst_is_
         if (it == server->live.end()) return NTT_ERR_NOT_FOUND;
         q = it->second;
         if (q->state != NTT_REQ_DONE) return q->state == NTT_REQ_FAILED ? q->status : NTT_ERR_BUSY;
-        if (!q->placed) return NTT_ERR_NOT_FOUND;       // auto-released
+        if (!q->placed || q->release_pending) return NTT_ERR_NOT_FOUND;   // auto-released or released
+        ++q->pins;   // the arena range stays allocated until this copy is done
     }
-    int rc = ck(hipMemcpy(dst, server->arena + q->off / sizeof(uint32_t), q->bytes,
-                          dst_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
-                "hipMemcpy(fetch)");
-    if (rc) return rc;
-    return release ? ntt_server_release(server, id) : NTT_OK;
+    const int rc = ck(hipMemcpy(dst, server->arena + q->off / sizeof(uint32_t), q->bytes,
+                                dst_is_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
+                      "hipMemcpy(fetch)");
+    std::lock_guard<std::mutex> g(server->mu);
+    --q->pins;
+    if (release && rc == NTT_OK) q->release_pending = true;
+    if (q->release_pending && q->pins == 0) release_locked(server, q);
+    return rc;
 }
 
 int ntt_server_release(ntt_server* server, uint64_t id) {
@@ -591,13 +615,10 @@ int ntt_server_release(ntt_server* server, uint64_t id) {
     if (it == server->live.end()) return NTT_ERR_NOT_FOUND;
     auto q = it->second;
     if (q->state != NTT_REQ_DONE && q->state != NTT_REQ_FAILED) return NTT_ERR_BUSY;
-    if (q->placed) {
-        server->alloc.release(q->off, q->bytes);
-        q->placed = false;
-        server->stats.arena_bytes = server->alloc.used();
-    }
-    server->live.erase(it);
-    server->cv_work.notify_one();
+    if (q->release_pending) return NTT_ERR_NOT_FOUND;
+    q->release_pending = true;
+    // a fetch still copying the result frees it when it finishes
+    if (q->pins == 0) release_locked(server, q);
     return NTT_OK;
 }
 
@@ -615,7 +636,6 @@ int ntt_server_prepare(ntt_server* server, int logN, uint32_t limbs, int domain)
     r.logN = logN;
     r.limbs = limbs;
     r.domain = domain;
-    std::lock_guard<std::mutex> g(server->mu);
     ntt_plan* P = nullptr;
     return plan_for(server, r, &P);
 }

@@ -93,6 +93,35 @@ def test_all_sizes_default_config(env, logN):
         assert np.array_equal(back, x)
 
 
+@pytest.mark.parametrize("logN", [18, 19, 20])
+def test_largest_sizes(env, logN):
+    """The sizes above the baseline workloads up to the largest accepted
+    (ntt_max_log_n): drop-in launchers (default split) and plans, both
+    domains, against the oracle; one size past it is refused."""
+    ntt_amd, gu, orc = env
+    assert ntt_amd.lib().ntt_max_log_n() == 20
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, 2)]
+    x = gu.random_operand(logN, moduli, logN)
+    ctx = ntt_amd.Context(0)
+    for domain in (1, 0):
+        _print("case large size", logN, domain)
+        exp = orc.forward(x, logN, moduli, domain)
+        fwd, back = run_dropin(env, logN, moduli, domain, x)
+        assert np.array_equal(fwd, exp), (logN, domain)
+        assert np.array_equal(back, x)
+        plan = ctx.plan(logN, moduli, domain)
+        A = gu.to_dev(x)
+        plan.forward(A)
+        gu.sync()
+        assert np.array_equal(gu.to_host(A).reshape(x.shape), exp)
+        plan.inverse(A)
+        gu.sync()
+        assert np.array_equal(gu.to_host(A).reshape(x.shape), x)
+    if logN == 20:
+        with pytest.raises(ntt_amd.NTTError):
+            ctx.plan(21, [tuple(m) for m in ntt_amd.generate_moduli(21, 1)])
+
+
 def _configs(logN):
     """A spread of valid phase configs for logN (both pass orders and shapes)."""
     out = []

@@ -178,3 +178,63 @@ def test_server_argument_errors(env):
     with pytest.raises(ntt_amd.NTTError):
         srv.wait(12345)                       # unknown id
     srv.close()
+
+
+def test_server_serving_stream_results(env):
+    """The served-stream workload itself (logN U{13..17}, L log-uniform on
+    [4, 64], direction Bernoulli(1/2), Poisson arrivals, seeded operands):
+    a 2000-request prefix submitted at its arrival times with results kept
+    (about 8 GB resident).  Every result is checked by the opposite transform
+    on the device against its regenerated seeded operand, and the first
+    request of every (logN, direction) against the oracle."""
+    import time
+    ntt_amd, gu, orc = env
+    from ntt_amd import workloads
+    reqs = workloads.serving_stream(0, 2000, rate_per_s=50000.0)
+    total = sum(4 * r.job.words for r in reqs)
+    ctx = ntt_amd.Context(0)
+    srv = ctx.server(hbm_budget_bytes=total + (256 << 20), slots=4, max_jobs_per_launch=64)
+    _print("case server-c5 submit", len(reqs), "requests", total >> 20, "MiB")
+    ids = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        while time.perf_counter() - t0 < r.arrival_s:
+            pass
+        ids.append(srv.submit(r.job.logN, r.job.limbs, r.job.direction, seed=r.seed))
+    for rid in ids:
+        srv.wait(rid)
+    st = srv.stats()
+    _print("case server-c5 done", st)
+    assert st["completed"] == len(reqs)
+    assert st["mean_jobs_per_launch"] > 1.0            # requests were batched
+    plans, qdev = {}, {}
+    oracle_done = set()
+    for k, (r, rid) in enumerate(zip(reqs, ids)):
+        logN, limbs, direction = r.job.logN, r.job.limbs, r.job.direction
+        shape = (logN, limbs)
+        m = workloads.moduli_for(logN, limbs)
+        if shape not in plans:
+            plans[shape] = ctx.plan(logN, m, ntt_amd.MONTGOMERY)
+            qdev[shape] = gu.to_dev(np.array([q for q, _ in m], np.uint32))
+        words = limbs << logN
+        res = ntt_amd.DeviceBuffer(4 * words)
+        srv.fetch_device(rid, res, release=False)
+        x = ntt_amd.DeviceBuffer(4 * words)
+        ntt_amd.device.fill_uniform(x, qdev[shape], limbs, logN, 1, seed=r.seed)
+        gu.sync()
+        xh = gu.to_host(x).reshape(limbs, 1 << logN)
+        if (logN, direction) not in oracle_done:
+            oracle_done.add((logN, direction))
+            exp = _oracle(orc, xh, logN, m, direction)
+            assert np.array_equal(gu.to_host(res).reshape(xh.shape), exp), f"request {k} vs oracle"
+        # the opposite transform brings the result back to the operand
+        if direction == ntt_amd.FORWARD:
+            plans[shape].inverse(res)
+        else:
+            plans[shape].forward(res)
+        gu.sync()
+        assert np.array_equal(gu.to_host(res).reshape(xh.shape), xh), \
+            f"request {k}: logN {logN} limbs {limbs} direction {direction}"
+        srv.release(rid)
+    assert len(oracle_done) == 10
+    srv.close()
