@@ -42,7 +42,8 @@ typedef enum {
     NTT_ERR_BUDGET = 8,             /* request larger than the server's HBM budget */
     NTT_ERR_NOT_FOUND = 9,          /* unknown request id */
     NTT_ERR_BUSY = 10,              /* queue full / try again */
-    NTT_ERR_NO_DEVICE = 11
+    NTT_ERR_NO_DEVICE = 11,
+    NTT_ERR_SCRATCH_LIMIT = 12      /* more jobs or limbs than ntt_batch_run_static's fixed tables hold */
 } ntt_status;
 
 typedef enum { NTT_DOMAIN_PLAIN = 0, NTT_DOMAIN_MONTGOMERY = 1 } ntt_domain;
@@ -371,6 +372,39 @@ int ntt_batch_set_data(ntt_batch* batch, size_t index, uint32_t* data);
 /* fused_groups: persistent fused launches; two_pass_jobs: jobs on two-pass
  * kernels; streams: concurrent streams used. */
 int ntt_batch_info(const ntt_batch* batch, int* fused_groups, int* two_pass_jobs, int* streams);
+
+/* ---------------------------------------------------------------------------
+ * Allocation-free batch launch.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under hipGraph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  It runs the same
+ * kernels as ntt_batch_run (one persistent fused kernel per (logN,
+ * direction) with 10 <= logN <= 16, one two-pass transform per other
+ * (logN, direction)), Montgomery variant, over jobs that bring the device
+ * tables of the reference's launchers (src/v04_montgomery.cu:603-658), and
+ * calls no hipMalloc or hipFree: the job tables those kernels read live in a
+ * fixed device array of the library (about 0.75 MiB per GPU, held until the
+ * process ends), sized for at most NTT_STATIC_MAX_JOBS jobs and
+ * NTT_STATIC_MAX_LIMBS limbs in total per call (NTT_ERR_SCRATCH_LIMIT
+ * beyond).  Every call on one GPU shares that array: calls that overlap on a
+ * device (other streams or threads) race and must be serialised by the
+ * caller.  Asynchronous on `stream`; canonical (not lazy) arithmetic.
+ * ------------------------------------------------------------------------- */
+#define NTT_STATIC_MAX_JOBS 1024
+#define NTT_STATIC_MAX_LIMBS 16384
+
+typedef struct {
+    uint32_t* data;               /* device operand, limbs x 2^logN words, in place, 16-byte aligned */
+    const uint32_t* twiddles;     /* device, limbs x 2^logN Montgomery-form table, limb-major:
+                                     psi^brv(i) forward, psi^-brv(i) inverse (v04:909-915) */
+    const uint32_t* moduli;       /* device, limbs: q */
+    const uint32_t* q_inv;        /* device, limbs: -q^-1 mod 2^32 */
+    const uint32_t* scale;        /* device, limbs: R^2 mod q forward, to_mont(N^-1) inverse */
+    uint32_t limbs;
+    int32_t logN;
+    int32_t direction;            /* ntt_direction */
+} ntt_table_job;
+
+int ntt_batch_run_static(const ntt_table_job* jobs, size_t njobs, void* stream);
 
 /* ---------------------------------------------------------------------------
  * Serving: a request loop with continuous batching.  No counterpart in the

@@ -40,7 +40,7 @@ STATUS = {
     0: "NTT_OK", 1: "NTT_ERR_INVALID_ARGUMENT", 2: "NTT_ERR_MODULUS_CONGRUENCE",
     3: "NTT_ERR_PRIMITIVE_ROOT", 4: "NTT_ERR_MODULUS_RANGE", 5: "NTT_ERR_CONFIG",
     6: "NTT_ERR_HIP", 7: "NTT_ERR_OUT_OF_MEMORY", 8: "NTT_ERR_BUDGET", 9: "NTT_ERR_NOT_FOUND",
-    10: "NTT_ERR_BUSY", 11: "NTT_ERR_NO_DEVICE",
+    10: "NTT_ERR_BUSY", 11: "NTT_ERR_NO_DEVICE", 12: "NTT_ERR_SCRATCH_LIMIT",
 }
 PLAIN, MONTGOMERY = 0, 1
 FORWARD, INVERSE = 0, 1
@@ -145,6 +145,13 @@ class BatchJob(ctypes.Structure):
                 ("logN", ctypes.c_int32), ("direction", ctypes.c_int32), ("domain", ctypes.c_int32)]
 
 
+class TableJob(ctypes.Structure):
+    """ntt_table_job: a job of ntt_batch_run_static (device pointers)."""
+    _fields_ = [("data", ctypes.c_void_p), ("twiddles", ctypes.c_void_p), ("moduli", ctypes.c_void_p),
+                ("q_inv", ctypes.c_void_p), ("scale", ctypes.c_void_p), ("limbs", ctypes.c_uint32),
+                ("logN", ctypes.c_int32), ("direction", ctypes.c_int32)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_size_t), ("slots", ctypes.c_int), ("max_jobs_per_launch", ctypes.c_int)]
 
@@ -247,6 +254,7 @@ _PROTOS = {
     "ntt_batch_replay": (_I, [_P, _P]),
     "ntt_batch_set_data": (_I, [_P, _SZ, _P]),
     "ntt_batch_info": (_I, [_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ntt_batch_run_static": (_I, [ctypes.POINTER(TableJob), _SZ, _P]),
     "ntt_server_create": (_I, [_P, _P, ctypes.POINTER(_P)]),
     "ntt_server_destroy": (_I, [_P]),
     "ntt_server_submit": (_I, [_P, _P, ctypes.POINTER(ctypes.c_uint64)]),
@@ -512,6 +520,15 @@ class Context:
 
     def server(self, hbm_budget_bytes: int = 64 << 30, slots: int = 4, max_jobs_per_launch: int = 256) -> "Server":
         return Server(self, hbm_budget_bytes, slots, max_jobs_per_launch)
+
+
+def batch_run_static(jobs, stream=None):
+    """ntt_batch_run_static: jobs are (data, twiddles, moduli, q_inv, scale,
+    limbs, logN, direction) tuples of device buffers (or addresses) and ints."""
+    arr = (TableJob * len(jobs))()
+    for k, (data, tw, q, qi, sc, limbs, logN, direction) in enumerate(jobs):
+        arr[k] = TableJob(_addr(data), _addr(tw), _addr(q), _addr(qi), _addr(sc), limbs, logN, direction)
+    _check(_lib.ntt_batch_run_static(arr, len(jobs), _stream(stream)), "ntt_batch_run_static")
 
 
 class Server:

@@ -202,6 +202,7 @@ const char* ntt_status_string(int s) {
         case NTT_ERR_NOT_FOUND: return "unknown request";
         case NTT_ERR_BUSY: return "busy";
         case NTT_ERR_NO_DEVICE: return "no HIP device";
+        case NTT_ERR_SCRATCH_LIMIT: return "more jobs or limbs than the allocation-free launch's fixed tables hold";
         default: return "unknown status";
     }
 }
