@@ -390,12 +390,12 @@ def main():
         log("ROUNDTRIP MISMATCH after the timed region")
 
     # ---- kernel durations of the engine in use (HIP events on the launch stream) ----
-    ebf, lzf = plan.fused_variant(B, ntt_amd.FORWARD)
-    ebi, _ = plan.fused_variant(B, ntt_amd.INVERSE)
+    ebf, lzf, lwf = plan.fused_variant_full(B, ntt_amd.FORWARD)
+    ebi, lzi, lwi = plan.fused_variant_full(B, ntt_amd.INVERSE)
     lazy = all(q < (1 << 30) for q, _ in moduli)   # two-pass kernels then use lazy reduction
     if engine_used == "fused":
-        kern = {f"fused_kernel<{logN}, false, {'true' if lzf else 'false'}, {ebf}, false, false>": lambda: plan.forward(pool, B, stream),
-                f"fused_kernel<{logN}, true, false, {ebi}, false, false>": lambda: plan.inverse(pool, B, stream)}
+        kern = {f"fused_kernel<{logN}, false, {'true' if lzf else 'false'}, {ebf}, false, false, {lwf}>": lambda: plan.forward(pool, B, stream),
+                f"fused_kernel<{logN}, true, {'true' if lzi else 'false'}, {ebi}, false, false, {lwi}>": lambda: plan.inverse(pool, B, stream)}
     else:
         mode = 2 if lazy else 0
         c = cfg.astuple()
@@ -572,7 +572,7 @@ def main():
                         f"({4 * words_per_op * B / 2**30:.2f} GiB, HBM-resident), batched launch",
             "logN": logN, "limbs": limbs, "batch": B,
             "engine": engine_used,
-            "fused_variant": {"forward": [ebf, lzf], "inverse": [ebi, False]} if engine_used == "fused" else None,
+            "fused_variant": {"forward": [ebf, lzf, lwf], "inverse": [ebi, lzi, lwi]} if engine_used == "fused" else None,
             "phase_config": cfg.astuple(),
             "chunk_operands": tuning["batch"]["chunk"] if tuning else 0,
             "parallelism": "single GPU",

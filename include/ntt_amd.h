@@ -210,10 +210,14 @@ int ntt_plan_set_engine(ntt_plan* plan, int engine);
  * `limbs` words, reduced mod q; NULL means 1.  Synchronous; do not call while
  * launches of this plan are in flight. */
 int ntt_plan_set_constants(ntt_plan* plan, const uint32_t* prologue, const uint32_t* epilogue);
-/* Workgroup shape of the fused engine for both directions: 7 = 512 threads x
- * 128 words each, 6 = 1024 threads x 64 words, 0 = the tuned choice
- * (default: 7 forward, 6 inverse). */
+/* Workgroup shape of the fused engine for both directions, per CU: 7 = 512
+ * threads x 128 words each, 6 = 1024 threads x 64 words, 0 = the tuned
+ * choice (default 6). */
 int ntt_plan_set_fused_shape(ntt_plan* plan, int eb);
+/* Words per fused workgroup, log2: 16 (one workgroup per CU, every fused
+ * size), 15 (two independent workgroups per CU, logN <= 15; larger sizes
+ * keep 16), 0 = default (16). */
+int ntt_plan_set_fused_words(ntt_plan* plan, int log2_words);
 /* On-the-fly twiddles (ntt_twiddle_mode) for the plan's fused-engine launches:
  * the compressed-twiddle scheme of Cheddar (msb table of every LsbSize-th
  * twiddle, NTT_cheddar.cu:1283-1289; MultiRadixNTT_OT / MultiRadixINTT_OT in
@@ -223,13 +227,14 @@ int ntt_plan_set_fused_shape(ntt_plan* plan, int eb);
  * every q < 2^30, else 6 canonical; inverse 6 canonical).  Two-pass launches
  * and the mod-down form read the table as before. */
 int ntt_plan_set_twiddle_mode(ntt_plan* plan, int mode);
-/* Arithmetic of the fused forward kernel: 1 = lazy reduction (used only when
- * every q < 2^30), 0 = canonical, -1 = the tuned choice (default lazy).  The
- * inverse fused kernel is always canonical.  Outputs are canonical either way. */
+/* Arithmetic of the fused kernels, both directions: 1 = lazy reduction (used
+ * only when every q < 2^30), 0 = canonical, -1 = the tuned choice (default
+ * lazy).  Outputs are canonical either way. */
 int ntt_plan_set_fused_arith(ntt_plan* plan, int lazy);
-/* The fused kernel variant (shape, lazy) a run of `count` operands uses in
- * `direction`. */
-int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction, int* eb, int* lazy);
+/* The fused kernel variant (shape, lazy, words per workgroup as log2) a run
+ * of `count` operands uses in `direction`; any output may be NULL. */
+int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction, int* eb, int* lazy,
+                               int* log2_words);
 /* Engine a launch of `count` operands would use (AUTO resolved). */
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count);
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream);
@@ -320,6 +325,9 @@ typedef struct {
     int fused_eb;               /* fused forward shape: 7 = 512 threads x 128 words, 6 = 1024 x 64 */
     int fused_eb_inv;           /* fused inverse shape */
     int fused_lazy;             /* fused forward arithmetic: 1 lazy, 0 canonical */
+    int fused_lazy_inv;         /* fused inverse arithmetic: 1 lazy, 0 canonical */
+    int fused_words_log2;       /* fused forward words per workgroup (log2): 16 or 15 */
+    int fused_words_log2_inv;   /* fused inverse words per workgroup (log2) */
 } ntt_tune_result;
 
 int ntt_autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs,

@@ -128,7 +128,9 @@ class TuneResult(ctypes.Structure):
                 ("inv_ms", ctypes.c_double), ("default_fwd_ms", ctypes.c_double),
                 ("default_inv_ms", ctypes.c_double), ("candidates", ctypes.c_int), ("engine", ctypes.c_int),
                 ("fused_fwd_ms", ctypes.c_double), ("fused_inv_ms", ctypes.c_double), ("fused_stagger", ctypes.c_int),
-                ("fused_eb", ctypes.c_int), ("fused_eb_inv", ctypes.c_int), ("fused_lazy", ctypes.c_int)]
+                ("fused_eb", ctypes.c_int), ("fused_eb_inv", ctypes.c_int), ("fused_lazy", ctypes.c_int),
+                ("fused_lazy_inv", ctypes.c_int), ("fused_words_log2", ctypes.c_int),
+                ("fused_words_log2_inv", ctypes.c_int)]
 
     def as_dict(self):
         return {"cfg": self.cfg.astuple(), "chunk": int(self.chunk), "fwd_ms": self.fwd_ms, "inv_ms": self.inv_ms,
@@ -136,7 +138,8 @@ class TuneResult(ctypes.Structure):
                 "candidates": self.candidates, "engine": {1: "two_pass", 2: "fused"}.get(self.engine, self.engine),
                 "fused_fwd_ms": self.fused_fwd_ms, "fused_inv_ms": self.fused_inv_ms,
                 "fused_stagger": self.fused_stagger, "fused_eb": self.fused_eb, "fused_eb_inv": self.fused_eb_inv,
-                "fused_lazy": self.fused_lazy}
+                "fused_lazy": self.fused_lazy, "fused_lazy_inv": self.fused_lazy_inv,
+                "fused_words_log2": self.fused_words_log2, "fused_words_log2_inv": self.fused_words_log2_inv}
 
 
 class BatchJob(ctypes.Structure):
@@ -237,10 +240,11 @@ _PROTOS = {
     "ntt_plan_run_pass": (_I, [_P, _P, _SZ, _I, _I, _P]),
     "ntt_plan_set_engine": (_I, [_P, _I]),
     "ntt_plan_set_fused_shape": (_I, [_P, _I]),
+    "ntt_plan_set_fused_words": (_I, [_P, _I]),
     "ntt_plan_set_fused_arith": (_I, [_P, _I]),
     "ntt_plan_set_twiddle_mode": (_I, [_P, _I]),
     "ntt_plan_set_constants": (_I, [_P, _PU32, _PU32]),
-    "ntt_plan_get_fused_variant": (_I, [_P, _SZ, _I, ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ntt_plan_get_fused_variant": (_I, [_P, _SZ, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
     "ntt_plan_get_engine": (_I, [_P, _SZ]),
     "ntt_autotune": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _SZ, _I, _I, ctypes.POINTER(TuneResult)]),
     "ntt_context_set_tuning": (_I, [_P, _I, _SZ, _PCFG, _SZ, _I]),
@@ -752,10 +756,14 @@ class Plan:
 
     def fused_variant(self, count: int = 1, direction: int = FORWARD):
         """(shape, lazy) of the fused kernel a run of `count` operands uses."""
-        eb, lz = _I(0), _I(0)
-        _check(_lib.ntt_plan_get_fused_variant(self.handle, count, direction, ctypes.byref(eb), ctypes.byref(lz)),
-               "ntt_plan_get_fused_variant")
-        return eb.value, bool(lz.value)
+        return self.fused_variant_full(count, direction)[:2]
+
+    def fused_variant_full(self, count: int = 1, direction: int = FORWARD):
+        """(shape, lazy, log2 words per workgroup) of the fused kernel."""
+        eb, lz, lw = _I(0), _I(0), _I(0)
+        _check(_lib.ntt_plan_get_fused_variant(self.handle, count, direction, ctypes.byref(eb), ctypes.byref(lz),
+                                               ctypes.byref(lw)), "ntt_plan_get_fused_variant")
+        return eb.value, bool(lz.value), lw.value
 
     def set_constants(self, prologue=None, epilogue=None):
         """Fused per-limb constant multiplies: forward computes NTT(prologue * x),
@@ -776,6 +784,9 @@ class Plan:
     def set_fused_arith(self, lazy: int):
         """Fused forward arithmetic: 1 lazy (when every q < 2^30), 0 canonical, -1 tuned."""
         _check(_lib.ntt_plan_set_fused_arith(self.handle, lazy), "ntt_plan_set_fused_arith")
+
+    def set_fused_words(self, log2_words: int):
+        _check(_lib.ntt_plan_set_fused_words(self.handle, log2_words), "ntt_plan_set_fused_words")
 
     def set_fused_shape(self, eb: int):
         """Fused workgroup shape, both directions: 7 (512 threads x 128 words), 6 (1024 x 64), 0 = tuned."""

@@ -182,8 +182,7 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
     size_t best_chunk = 0;
     double def_fwd = 0, def_inv = 0, fused_fwd = 0, fused_inv = 0;
     int engine = NTT_ENGINE_TWO_PASS;
-    int fused_eb_fwd = FUSED_DEFAULT_EB_FWD, fused_eb_inv = FUSED_DEFAULT_EB_INV;
-    bool fused_lazy = P.lazy;
+    FusedVariant vf, vi;   // fused variants per direction
     int stagger = 0;
     if (rc == NTT_OK) {
         // chunking of multi-operand launches (whole transforms, both passes)
@@ -208,32 +207,29 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
             }
         }
         // the fused limb kernel where it applies: each direction on its own,
-        // over both workgroup shapes, and for the forward direction both
-        // arithmetics when every q < 2^30
+        // over both window widths, both arithmetics when every q < 2^30, and
+        // for logN <= 15 both workgroup sizes (2^16 words, one per CU; 2^15
+        // words, two per CU)
         if (rc == NTT_OK && fused_supported(logN)) {
             double bf = 0, bi = 0;
-            for (int eb = 7; rc == NTT_OK && eb >= 6; --eb) {
-                for (int lz = P.lazy ? 1 : 0; rc == NTT_OK && lz >= 0; --lz) {
-                    double f = 0;
-                    rc = time_steady(s, reps + 2,
-                                     [&] { return plan_launch_fused(&P, buf, count, false, s, eb, lz != 0, 0); }, &f);
-                    ++ncand;
-                    if (rc == NTT_OK && (bf == 0 || f < bf)) {
-                        bf = f;
-                        fused_eb_fwd = eb;
-                        fused_lazy = lz != 0;
-                    }
-                }
-                double i = 0;
-                if (rc == NTT_OK)
-                    rc = time_steady(s, reps + 2,
-                                     [&] { return plan_launch_fused(&P, buf, count, true, s, eb, false, 0); }, &i);
-                ++ncand;
-                if (rc == NTT_OK && (bi == 0 || i < bi)) {
-                    bi = i;
-                    fused_eb_inv = eb;
-                }
-            }
+            for (int logw = 16; rc == NTT_OK && logw >= (logN <= 15 ? 15 : 16); --logw)
+                for (int eb = 7; rc == NTT_OK && eb >= 6; --eb)
+                    for (int lz = P.lazy ? 1 : 0; rc == NTT_OK && lz >= 0; --lz)
+                        for (int inv = 0; rc == NTT_OK && inv < 2; ++inv) {
+                            FusedVariant v;
+                            v.eb = eb;
+                            v.lazy = lz != 0;
+                            v.logw = logw;
+                            double t = 0;
+                            rc = time_steady(s, reps + 2,
+                                             [&] { return plan_launch_fused(&P, buf, count, inv != 0, s, v, 0); }, &t);
+                            ++ncand;
+                            double& b = inv ? bi : bf;
+                            if (rc == NTT_OK && (b == 0 || t < b)) {
+                                b = t;
+                                (inv ? vi : vf) = v;
+                            }
+                        }
             fused_fwd = bf;
             fused_inv = bi;
             // the two-pass finalist, timed the same way
@@ -276,9 +272,12 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
     TunedEntry e;
     e.cfg = best;
     e.engine = engine;
-    e.fused_eb_fwd = fused_eb_fwd;
-    e.fused_eb_inv = fused_eb_inv;
-    e.fused_lazy = fused_lazy;
+    e.fused_eb_fwd = vf.eb;
+    e.fused_eb_inv = vi.eb;
+    e.fused_lazy = vf.lazy;
+    e.fused_lazy_inv = vi.lazy;
+    e.fused_logw_fwd = vf.logw;
+    e.fused_logw_inv = vi.logw;
     e.stagger = stagger;
     e.chunk = best_chunk;
     e.fwd_ms = best_fwd;
@@ -299,9 +298,12 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
         out->fused_fwd_ms = fused_fwd;
         out->fused_inv_ms = fused_inv;
         out->fused_stagger = stagger;
-        out->fused_eb = fused_eb_fwd;
-        out->fused_eb_inv = fused_eb_inv;
-        out->fused_lazy = fused_lazy ? 1 : 0;
+        out->fused_eb = vf.eb;
+        out->fused_eb_inv = vi.eb;
+        out->fused_lazy = vf.lazy ? 1 : 0;
+        out->fused_lazy_inv = vi.lazy ? 1 : 0;
+        out->fused_words_log2 = vf.logw;
+        out->fused_words_log2_inv = vi.logw;
     }
     return NTT_OK;
 }
@@ -348,15 +350,16 @@ int ntt_context_save_tuning(const ntt_context* ctx, const char* path) {
     FILE* f = std::fopen(path, "w");
     if (!f) return NTT_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> g(ctx->tune_mu);
-    std::fprintf(f, "# ntt_amd tuning v2: logN limbs_log2 chunk r s w x4 fwd_ms inv_ms engine fused_eb_fwd "
-                    "fused_eb_inv fused_lazy stagger\n");
+    std::fprintf(f, "# ntt_amd tuning v3: logN limbs_log2 chunk r s w x4 fwd_ms inv_ms engine fused_eb_fwd "
+                    "fused_eb_inv fused_lazy stagger fused_lazy_inv fused_logw_fwd fused_logw_inv\n");
     for (auto& kv : ctx->tuned) {
         const TunedEntry& e = kv.second;
         const ntt_phase_config* p = &e.cfg.ntt_phase1;
         std::fprintf(f, "%d %d %zu", kv.first.first, kv.first.second, e.chunk);
         for (int i = 0; i < 4; ++i) std::fprintf(f, " %d %d %d", p[i].radix_stages, p[i].stage_merging, p[i].warp_batching);
-        std::fprintf(f, " %.6f %.6f %d %d %d %d %d\n", e.fwd_ms, e.inv_ms, e.engine, e.fused_eb_fwd, e.fused_eb_inv,
-                     e.fused_lazy ? 1 : 0, e.stagger);
+        std::fprintf(f, " %.6f %.6f %d %d %d %d %d %d %d %d\n", e.fwd_ms, e.inv_ms, e.engine, e.fused_eb_fwd,
+                     e.fused_eb_inv, e.fused_lazy ? 1 : 0, e.stagger, e.fused_lazy_inv ? 1 : 0, e.fused_logw_fwd,
+                     e.fused_logw_inv);
     }
     std::fclose(f);
     return NTT_OK;
@@ -374,9 +377,10 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         size_t chunk;
         double fm = 0, im = 0;
         int eng = NTT_ENGINE_TWO_PASS, febf = FUSED_DEFAULT_EB_FWD, febi = FUSED_DEFAULT_EB_INV, lz = 1, stag = 0;
-        int n = std::sscanf(line, "%d %d %zu %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d %d %d %d %d", &logN, &lb,
-                            &chunk, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10],
-                            &v[11], &fm, &im, &eng, &febf, &febi, &lz, &stag);
+        int lzi = 1, lwf = 16, lwi = 16;
+        int n = std::sscanf(line, "%d %d %zu %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d %d %d %d %d %d %d %d",
+                            &logN, &lb, &chunk, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9],
+                            &v[10], &v[11], &fm, &im, &eng, &febf, &febi, &lz, &stag, &lzi, &lwf, &lwi);
         if (n < 15) { rc = NTT_ERR_INVALID_ARGUMENT; break; }
         TunedEntry e;
         ntt_phase_config* p = &e.cfg.ntt_phase1;
@@ -387,6 +391,9 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         e.fused_eb_fwd = febf == 6 ? 6 : 7;
         e.fused_eb_inv = febi == 6 ? 6 : 7;
         e.fused_lazy = lz != 0;
+        e.fused_lazy_inv = lzi != 0;
+        e.fused_logw_fwd = lwf == 15 && logN <= 15 ? 15 : 16;
+        e.fused_logw_inv = lwi == 15 && logN <= 15 ? 15 : 16;
         e.stagger = stag < 0 ? 0 : (stag > 64 ? 64 : stag);
         e.fwd_ms = fm;
         e.inv_ms = im;

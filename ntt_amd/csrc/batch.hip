@@ -27,9 +27,10 @@
 #include "phase_kernels.hpp"
 
 namespace ntt {
-int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
+int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
+                 int logw = 16);
 int fused_prepare();
-uint32_t fused_items(int logN, size_t limbs);
+uint32_t fused_items(int logN, size_t limbs, int logw = 16);
 }  // namespace ntt
 
 using namespace ntt;
@@ -45,8 +46,7 @@ struct PlanDeleter {
 struct FusedGroup {
     int n = 0;
     bool inverse = false;
-    int eb = FUSED_DEFAULT_EB_FWD;
-    bool lazy = false;              // lazy forward kernel (every job's q < 2^30)
+    FusedVariant v;                 // kernel variant (lazy only when every job's q < 2^30)
     size_t desc_begin = 0;          // first descriptor in the batch's table
     size_t njobs = 0;
     uint32_t items = 0;
@@ -136,7 +136,7 @@ int enqueue(ntt_batch* b, hipStream_t stream) {
                 p.total_items = g.items;
                 p.counter = nullptr;          // static round-robin: nothing to reset between replays
                 p.stagger = 0;
-                rc = launch_fused(g.n, g.inverse, g.eb, g.lazy, p, s, 0);
+                rc = launch_fused(g.n, g.inverse, g.v.eb, g.v.lazy, p, s, 0, g.v.logw);
             } else {
                 const TwoPassGroup& g = b->tp_groups[(size_t)u.tp_group];
                 TransformLaunch L{};
@@ -232,6 +232,8 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         g.desc_begin = b->desc.size();
         g.njobs = kv.second.size();
         size_t total_limbs = 0;
+        for (size_t i : kv.second) total_limbs += b->jobs[i].limbs;
+        g.v = fused_variant_for(ctx, g.n, total_limbs, g.inverse, lazy_capable);
         uint64_t items = 0;
         for (size_t i : kv.second) {
             const ntt_batch_job& j = b->jobs[i];
@@ -246,16 +248,14 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
             fj.item_begin = (uint32_t)items;
             fj.n = (uint16_t)g.n;
             fj.inverse = g.inverse ? 1 : 0;
-            items += fused_items(g.n, j.limbs);
+            items += fused_items(g.n, j.limbs, g.v.logw);
             if (items > 0xffffffffull) return NTT_ERR_INVALID_ARGUMENT;
             b->job_group[i] = (int)b->groups.size();
             b->job_desc[i] = b->desc.size();
             b->desc.push_back(fj);
-            total_limbs += j.limbs;
             g.work += 0.5 * (double)((size_t)1 << g.n) * g.n * j.limbs;
         }
         g.items = (uint32_t)items;
-        fused_variant_for(ctx, g.n, total_limbs, g.inverse, lazy_capable, &g.eb, &g.lazy);
         units.push_back({(int)b->groups.size(), -1, g.work});
         b->groups.push_back(g);
     }

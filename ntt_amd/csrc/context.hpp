@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/ntt_amd.h"
+#include "fused_types.hpp"
 #include "launch.hpp"
 
 namespace ntt {
@@ -35,8 +36,10 @@ struct LimbConst;
 struct TunedEntry {
     ntt_transform_config cfg;
     int engine = NTT_ENGINE_TWO_PASS;   // NTT_ENGINE_TWO_PASS or NTT_ENGINE_FUSED
-    int fused_eb_fwd = 7, fused_eb_inv = 6;   // fused engine: window width (6 or 7) per direction
-    bool fused_lazy = true;             // fused engine: lazy forward arithmetic (when every q < 2^30)
+    int fused_eb_fwd = FUSED_DEFAULT_EB_FWD, fused_eb_inv = FUSED_DEFAULT_EB_INV;   // fused window width (6 or 7)
+    bool fused_lazy = true;             // fused forward arithmetic: lazy (when every q < 2^30)
+    bool fused_lazy_inv = true;         // fused inverse arithmetic: lazy (when every q < 2^30)
+    int fused_logw_fwd = 16, fused_logw_inv = 16;   // fused words per workgroup (log2) per direction
     int stagger = 0;                    // fused engine: start stagger (FusedParams::stagger)
     size_t chunk = 0;          // operands per chunk, 0 = whole launch
     double fwd_ms = 0, inv_ms = 0;
@@ -63,6 +66,7 @@ struct ntt_plan {
     int engine = NTT_ENGINE_AUTO;
     int fused_eb = 0;                              // fused workgroup shape: 6, 7, or 0 = tuned/default
     int fused_arith = -1;                          // fused forward arithmetic: 1 lazy, 0 canonical, -1 tuned/default
+    int fused_logw = 0;                            // fused words per workgroup (log2): 15, 16, or 0 = tuned/default
     bool lazy = false;                             // every q < 2^30: lazy-reduction kernels
     size_t chunk = 0;
     void* d_const = nullptr;                       // one allocation holding the arrays below
@@ -93,14 +97,14 @@ TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool i
 int log2_bucket(size_t v);
 // Engine a launch of `count` operands runs on (resolves NTT_ENGINE_AUTO).
 int plan_engine(const ntt_plan* P, size_t count);
-int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream, int eb,
-                      bool lazy, int stagger, const uint32_t* src = nullptr, const uint32_t* sub = nullptr);
+int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
+                      const FusedVariant& v, int stagger, const uint32_t* src = nullptr, const uint32_t* sub = nullptr);
 // The fused kernel variant a run of `count` operands uses in `inverse` direction.
-void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, int* eb, bool* lazy, int* stagger);
+void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, FusedVariant* v, int* stagger);
 // Fused variant for a launch of `total_limbs` limbs of size 2^logN from the
 // context's tuning (or the defaults); lazy only if the caller's moduli allow.
-void fused_variant_for(const ntt_context* ctx, int logN, size_t total_limbs, bool inverse, bool lazy_capable,
-                       int* eb, bool* lazy);
+FusedVariant fused_variant_for(const ntt_context* ctx, int logN, size_t total_limbs, bool inverse,
+                               bool lazy_capable);
 bool fused_supported(int logN);
 int device_cus();
 }  // namespace ntt

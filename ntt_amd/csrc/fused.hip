@@ -27,8 +27,8 @@ int device_cus() {
 
 bool fused_supported(int logN) { return logN >= FUSED_MIN_LOG_N && logN <= FUSED_MAX_LOG_N; }
 
-uint32_t fused_items(int logN, size_t limbs) {
-    const size_t lpw = (size_t)1 << (16 - logN);
+uint32_t fused_items(int logN, size_t limbs, int logw) {
+    const size_t lpw = (size_t)1 << (logw - logN);
     return (uint32_t)((limbs + lpw - 1) / lpw);
 }
 
@@ -37,6 +37,7 @@ FusedFn fused_fn_eb6(int n, bool inv, bool lazy);   // fused_eb6.hip
 FusedFn fused_fn_eb7(int n, bool inv, bool lazy);   // fused_eb7.hip
 FusedFn fused_fn_md(int n, bool lazy);               // fused_md.hip
 FusedFn fused_fn_ot(int n, bool inv, bool lazy);     // fused_ot.hip
+FusedFn fused_fn_w15(int n, bool inv, int eb, bool lazy);   // fused_w15.hip
 
 // Window width of the on-the-fly twiddle kernels.
 static int ot_eb(bool inv, bool lazy) { return !inv && lazy ? 7 : 6; }
@@ -44,7 +45,9 @@ static int ot_eb(bool inv, bool lazy) { return !inv && lazy ? 7 : 6; }
 // Window width of the mod-down kernels: 7 with lazy arithmetic, 6 canonical.
 static int md_eb(bool lazy) { return lazy ? 7 : 6; }
 
-static FusedFn fused_fn(int n, bool inv, int eb, bool lazy) {
+static FusedFn fused_fn(int n, bool inv, int eb, bool lazy, int logw = 16) {
+    if (logw == 15) return fused_fn_w15(n, inv, eb, lazy);
+    if (logw != 16) return nullptr;
     return eb == 6 ? fused_fn_eb6(n, inv, lazy) : fused_fn_eb7(n, inv, lazy);
 }
 
@@ -69,6 +72,14 @@ int fused_prepare() {
                     attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
             }
+        const int lds15 = (int)(fused_lds_words(15) * sizeof(uint32_t));
+        for (int m = FUSED_MIN_LOG_N; m <= 15 && attr_err == hipSuccess; ++m)
+            for (int d = 0; d < 8 && attr_err == hipSuccess; ++d) {
+                FusedFn f = fused_fn(m, d & 1, 6 + ((d >> 1) & 1), d >= 4, 15);
+                if (f)
+                    attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, lds15);
+            }
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
             for (int lz = 0; lz < 2 && attr_err == hipSuccess; ++lz) {
                 FusedFn f = fused_fn_md(m, lz == 1);
@@ -80,17 +91,19 @@ int fused_prepare() {
     return attr_err == hipSuccess ? NTT_OK : record_hip_error(attr_err, "hipFuncSetAttribute(fused)");
 }
 
-// One launch of fused kernel `fn` (window width eb) over the items of p.
-static int launch_fused_fn(FusedFn fn, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit) {
+// One launch of fused kernel `fn` (window width eb, 2^logw words per
+// workgroup, 2^(16-logw) workgroups per CU) over the items of p.
+static int launch_fused_fn(FusedFn fn, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
+                           int logw = 16) {
     int rc = fused_prepare();
     if (rc) return rc;
-    const size_t lds = FUSED_LDS_WORDS * sizeof(uint32_t);
-    unsigned grid = (unsigned)device_cus();
+    const size_t lds = fused_lds_words(logw) * sizeof(uint32_t);
+    unsigned grid = (unsigned)device_cus() << (16 - logw);
     if (grid_limit && grid_limit < grid) grid = grid_limit;
     if (p.total_items < grid) grid = p.total_items;
     hipError_t stale = hipGetLastError();
     if (stale != hipSuccess) record_hip_error(stale, "error pending before launch (cleared)");
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(1u << (16 - eb)), lds, stream, p);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(1u << (logw - eb)), lds, stream, p);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? NTT_OK : record_hip_error(e, "fused kernel launch");
 }
@@ -98,19 +111,19 @@ static int launch_fused_fn(FusedFn fn, int eb, const FusedParams& p, hipStream_t
 // Launches the persistent kernel of size 2^n / direction `inv`; every job in
 // p has that size and direction.
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream,
-                 unsigned grid_limit) {
+                 unsigned grid_limit, int logw) {
     if (p.total_items == 0) return NTT_OK;
     if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
-    FusedFn fn = fused_fn(n, inv, eb, lazy);
+    FusedFn fn = fused_fn(n, inv, eb, lazy, logw);
     if (!fn) return NTT_ERR_CONFIG;
-    return launch_fused_fn(fn, eb, p, stream, grid_limit);
+    return launch_fused_fn(fn, eb, p, stream, grid_limit, logw);
 }
 
 // `count` operands of a plan as one job (limb l uses table l % limbs), read
 // from `src` (null: in place) and written to A; with `sub` non-null, the
 // forward mod-down form (the plan's per-limb multipliers).
-int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream, int eb,
-                      bool lazy, int stagger, const uint32_t* src, const uint32_t* sub) {
+int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
+                      const FusedVariant& v, int stagger, const uint32_t* src, const uint32_t* sub) {
     if (!fused_supported(P->logN)) return NTT_ERR_CONFIG;
     if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(sub)) & 15)
         return NTT_ERR_INVALID_ARGUMENT;
@@ -131,7 +144,8 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.job0.inverse = inverse ? 1 : 0;
     p.jobs = nullptr;
     p.njobs = 0;
-    p.total_items = fused_items(P->logN, limbs);
+    const bool w15 = v.logw == 15 && !P->twiddle_ot && !sub && P->logN <= 15;
+    p.total_items = fused_items(P->logN, limbs, w15 ? 15 : 16);
     p.counter = nullptr;
     p.stagger = stagger < 0 ? 0u : (uint32_t)stagger;
     if (P->twiddle_ot && !sub) {
@@ -149,8 +163,8 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
         if (p.total_items == 0) return NTT_OK;
         return launch_fused_fn(fn, md_eb(lz), p, stream, 0);
     }
-    // lazy reduction needs every q < 2^30 and exists for the forward kernels
-    return launch_fused(P->logN, inverse, eb, lazy && P->lazy && !inverse, p, stream, 0);
+    // lazy reduction needs every q < 2^30
+    return launch_fused(P->logN, inverse, v.eb, v.lazy && P->lazy, p, stream, 0, w15 ? 15 : 16);
 }
 
 }  // namespace ntt

@@ -1,6 +1,6 @@
 // Instantiations of the fused limb kernel with window width 7
 // (512 threads x 128 words per thread): canonical arithmetic in
-// both directions, lazy reduction (q < 2^30) for the forward transform.  One
+// both directions, and lazy reduction (q < 2^30) in both directions.  One
 // translation unit per shape, so the two compile in parallel.
 #include <hip/hip_runtime.h>
 
@@ -11,7 +11,18 @@ namespace ntt {
 FusedFn fused_fn_eb7(int n, bool inv, bool lazy) {
     constexpr int EB = 7;
     if (lazy) {
-        if (inv) return nullptr;
+        if (inv) {
+            switch (n) {
+                case 10: return fused_kernel<10, true, true, EB>;
+                case 11: return fused_kernel<11, true, true, EB>;
+                case 12: return fused_kernel<12, true, true, EB>;
+                case 13: return fused_kernel<13, true, true, EB>;
+                case 14: return fused_kernel<14, true, true, EB>;
+                case 15: return fused_kernel<15, true, true, EB>;
+                case 16: return fused_kernel<16, true, true, EB>;
+                default: return nullptr;
+            }
+        }
         switch (n * 2) {
         case 20: return fused_kernel<10, false, true, EB>;
         case 22: return fused_kernel<11, false, true, EB>;

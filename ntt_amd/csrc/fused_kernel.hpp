@@ -1,10 +1,13 @@
 // Fused limb-per-workgroup negacyclic NTT / INTT for gfx950.
 //
-// One workgroup (one per CU: it takes 132 KiB of the CU's 160 KiB LDS) holds
-// 64K words in registers: 2^(16-n) limbs of N = 2^n words.  With window width
-// EB, a thread holds 2^EB words and the workgroup has 2^(16-EB) threads:
-// EB = 6 is 1024 threads x 64 words (128 registers per thread available),
-// EB = 7 is 512 threads x 128 words (256 registers per thread available).  Every stage of the transform
+// A workgroup holds 2^LOGW words in registers: 2^(LOGW-n) limbs of N = 2^n
+// words.  LOGW = 16: one workgroup per CU (it takes 132 KiB of the CU's
+// 160 KiB LDS); LOGW = 15: two independent workgroups per CU (66 KiB of LDS
+// each), so that one can compute while the other loads or stores.  With
+// window width EB, a thread holds 2^EB words and the workgroup has
+// 2^(LOGW-EB) threads; per CU, EB = 6 is 1024 threads x 64 words (128
+// registers per thread available), EB = 7 is 512 threads x 128 words (256
+// registers per thread available).  Every stage of the transform
 // runs from registers; between rounds the words move through LDS in two
 // halves.  Each limb is read from and written to HBM exactly once, instead of
 // once per pass as in the two-pass kernels (src/v04_montgomery.cu:196-565 is
@@ -40,8 +43,6 @@
 namespace ntt {
 
 namespace fused {
-
-constexpr uint32_t LDS_WORDS = 32768 + 1024;
 
 
 __host__ __device__ constexpr int nrounds(int n, int eb) { return n <= 2 * eb - 1 ? 2 : 3; }
@@ -178,8 +179,8 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
                 uint32_t w;
                 if constexpr (OTS)
                     w = mont_mul(tw[ot_lo + (uint32_t)(x >> (j + 1))], ot_msb, q, q_inv);
-                else
-                    w = tw[(TB << (EB - 1 - j)) + (uint32_t)(x >> (j + 1))];
+                else   // 32-bit byte offset: SGPR table base plus a VGPR offset per load
+                    w = *(gcu32*)((gcu8*)tw + (((TB << (EB - 1 - j)) + (uint32_t)(x >> (j + 1))) << 2));
                 if (LAZY) {
                     if (INV)
                         gs_butterfly_lazy(v[a], v[b], w, q, q_inv);
@@ -201,21 +202,29 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[1 << EB], uint32_t* L, ui
     constexpr int S = sigma(n, INV, K, EB);
     constexpr int LOA = win_lo(n, INV, K, EB), LOB = win_lo(n, INV, K + 1, EB);
     constexpr int JA = S - LOA, JB = S - LOB;
-    uint32_t* const LA = L + lds_ubase<S, LOA, EB>(opaque(u));
-    uint32_t* const LB = L + lds_ubase<S, LOB, EB>(opaque(u));
+    // LDS accesses take a base register plus an immediate byte offset below
+    // 64 KiB; words further out go through a second and third base register
+    // (one per 16K words), made here and dead after the exchange, instead of
+    // one address register per word kept live by the compiler.
+    constexpr uint32_t WIN = 16384;
+    const uint32_t ua = lds_ubase<S, LOA, EB>(opaque(u)), ub = lds_ubase<S, LOB, EB>(opaque(u));
+    uint32_t* const LA[3] = {L + ua, L + opaque(ua + WIN), L + opaque(ua + 2 * WIN)};
+    uint32_t* const LB[3] = {L + ub, L + opaque(ub + WIN), L + opaque(ub + 2 * WIN)};
     static_for<2>([&](auto H) __attribute__((always_inline)) {
         constexpr int h = decltype(H)::value;
         if (valid) {
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
-                if constexpr (((x >> JA) & 1) == h) LA[LDS_OFF<S, LOA, x, EB>] = v[REG<n, INV, K, x, EB>];
+                constexpr uint32_t o = LDS_OFF<S, LOA, x, EB>;
+                if constexpr (((x >> JA) & 1) == h) LA[o / WIN][o % WIN] = v[REG<n, INV, K, x, EB>];
             });
         }
         __syncthreads();
         if (valid) {
             static_for<E>([&](auto X) __attribute__((always_inline)) {
                 constexpr int x = decltype(X)::value;
-                if constexpr (((x >> JB) & 1) == h) v[REG<n, INV, K + 1, x, EB>] = LB[LDS_OFF<S, LOB, x, EB>];
+                constexpr uint32_t o = LDS_OFF<S, LOB, x, EB>;
+                if constexpr (((x >> JB) & 1) == h) v[REG<n, INV, K + 1, x, EB>] = LB[o / WIN][o % WIN];
             });
         }
         __syncthreads();
@@ -319,6 +328,11 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     }
 
     // ---- store (last round layout) ----
+    // The words pass through an empty asm first: otherwise the compiler sinks
+    // the whole last round into the conditional store block below, where its
+    // schedule interleaves the round's products and spills.
+#pragma unroll
+    for (int x = 0; x < E; ++x) asm volatile("" : "+v"(v[x]));
     if (valid) {
         if (INV && (c.flags & 1)) {
 #pragma unroll
@@ -372,12 +386,16 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 }  // namespace fused
 
 // Persistent loop over work items of jobs of one size 2^n and one direction;
-// an item is 2^(16-n) consecutive limbs of one job.  Every thread of the
+// an item is 2^(LOGW-n) consecutive limbs of one job.  Every thread of the
 // workgroup runs every barrier of every item.
 // MD: forward mod-down form, storing (sub - NTT) * post (FusedJob).
 // OT: on-the-fly twiddles in the low-bit round (stages).
-template <int n, bool INV, bool LAZY, int EB, bool MD = false, bool OT = false>
-__global__ __launch_bounds__(1 << (16 - EB), 1) void fused_kernel(FusedParams p) {
+// The register budget is that of 2^(16-EB) threads per CU (2^(8-EB) waves
+// per SIMD) for either LOGW.
+template <int n, bool INV, bool LAZY, int EB, bool MD = false, bool OT = false, int LOGW = 16>
+__global__ __launch_bounds__(1 << (LOGW - EB)) __attribute__((amdgpu_waves_per_eu(1 << (8 - EB))))
+void fused_kernel(FusedParams p) {
+    static_assert(n <= LOGW && LOGW >= 15 && LOGW <= 16, "a workgroup holds whole limbs of at most 2^LOGW words");
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_next;
     uint32_t iter = 0;
@@ -406,7 +424,7 @@ __global__ __launch_bounds__(1 << (16 - EB), 1) void fused_kernel(FusedParams p)
             job = p.jobs[lo];
         }
         job = fused::uniform_job(job);
-        fused::run_item<n, INV, LAZY, EB, MD, OT>(job, (item - job.item_begin) << (16 - n), lds);
+        fused::run_item<n, INV, LAZY, EB, MD, OT>(job, (item - job.item_begin) << (LOGW - n), lds);
     }
 }
 

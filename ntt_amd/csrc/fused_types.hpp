@@ -7,13 +7,24 @@
 
 namespace ntt {
 
-// LDS of the fused kernel: half of the workgroup's 64K words plus padding.
-constexpr uint32_t FUSED_LDS_WORDS = 32768 + 1024;
-// Default window widths (7: 512 threads x 128 words; 6: 1024 threads x 64
-// words) per direction, the measured faster ones at N = 2^16 on MI355X; the
-// autotuner overrides them per launch size.
-constexpr int FUSED_DEFAULT_EB_FWD = 7;
+// LDS of the fused kernel: half of the workgroup's 2^logw words plus one
+// padding word per 32.
+constexpr uint32_t FUSED_LDS_WORDS = 32768 + 1024;   // logw = 16
+constexpr uint32_t fused_lds_words(int logw) { return (1u << (logw - 1)) + (1u << (logw - 6)); }
+// Default window widths (7: 512 threads x 128 words per CU; 6: 1024 threads
+// x 64 words) per direction, the measured faster ones at N = 2^16 on MI355X
+// (with lazy arithmetic in both directions where every q < 2^30); the
+// autotuner overrides them per launch size, together with the arithmetic and
+// the words per workgroup.
+constexpr int FUSED_DEFAULT_EB_FWD = 6;
 constexpr int FUSED_DEFAULT_EB_INV = 6;
+
+// The fused kernel variant of one launch direction.
+struct FusedVariant {
+    int eb = 6;          // window width: 6 or 7
+    bool lazy = false;   // lazy reduction (every q < 2^30)
+    int logw = 16;       // words per workgroup (log2): 16, or 15 (logN <= 15)
+};
 
 // On-the-fly twiddles: a cached table of N = 2^n words (n >= 10) is followed
 // by N / 64 words msb[h] = tw[64 h], so tw[i] = mont_mul(tw[i % 64], msb[i / 64])

@@ -170,26 +170,36 @@ int plan_engine(const ntt_plan* P, size_t count) {
     return fused_supported(P->logN) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
 }
 
-void fused_variant_for(const ntt_context* ctx, int logN, size_t total_limbs, bool inverse, bool lazy_capable,
-                       int* eb, bool* lazy) {
-    TunedEntry t;
-    const bool tuned = ctx->tuned_lookup(logN, total_limbs, &t);
-    *eb = tuned ? (inverse ? t.fused_eb_inv : t.fused_eb_fwd) : (inverse ? FUSED_DEFAULT_EB_INV : FUSED_DEFAULT_EB_FWD);
-    *lazy = (tuned ? t.fused_lazy : true) && lazy_capable && !inverse;
+// Tuned (or default) variant of one direction; 32K-word workgroups only
+// where whole limbs fit them (logN <= 15).
+static FusedVariant variant_of(const TunedEntry* t, int logN, bool inverse, bool lazy_capable) {
+    FusedVariant v;
+    v.eb = t ? (inverse ? t->fused_eb_inv : t->fused_eb_fwd) : (inverse ? FUSED_DEFAULT_EB_INV : FUSED_DEFAULT_EB_FWD);
+    v.lazy = (t ? (inverse ? t->fused_lazy_inv : t->fused_lazy) : true) && lazy_capable;
+    v.logw = t ? (inverse ? t->fused_logw_inv : t->fused_logw_fwd) : 16;
+    if (logN > 15) v.logw = 16;
+    return v;
 }
 
-void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, int* eb, bool* lazy, int* stagger) {
+FusedVariant fused_variant_for(const ntt_context* ctx, int logN, size_t total_limbs, bool inverse,
+                               bool lazy_capable) {
+    TunedEntry t;
+    const bool tuned = ctx->tuned_lookup(logN, total_limbs, &t);
+    return variant_of(tuned ? &t : nullptr, logN, inverse, lazy_capable);
+}
+
+void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, FusedVariant* v, int* stagger) {
     TunedEntry t;
     const bool tuned = !P->cfg_fixed && P->ctx->tuned_lookup(P->logN, P->limbs * count, &t);
-    *eb = P->fused_eb ? P->fused_eb
-                      : (tuned ? (inverse ? t.fused_eb_inv : t.fused_eb_fwd)
-                               : (inverse ? FUSED_DEFAULT_EB_INV : FUSED_DEFAULT_EB_FWD));
-    const bool want_lazy = P->fused_arith >= 0 ? P->fused_arith == 1 : (tuned ? t.fused_lazy : true);
-    *lazy = want_lazy && P->lazy && !inverse;   // lazy kernels: forward only, every q < 2^30
+    *v = variant_of(tuned ? &t : nullptr, P->logN, inverse, P->lazy);
+    if (P->fused_eb) v->eb = P->fused_eb;
+    if (P->fused_arith >= 0) v->lazy = P->fused_arith == 1 && P->lazy;
+    if (P->fused_logw) v->logw = P->logN <= 15 ? P->fused_logw : 16;
     *stagger = tuned ? t.stagger : 0;
     if (P->twiddle_ot) {   // fixed shapes of the on-the-fly twiddle kernels
-        *lazy = P->lazy && !inverse;
-        *eb = *lazy ? 7 : 6;
+        v->lazy = P->lazy && !inverse;
+        v->eb = v->lazy ? 7 : 6;
+        v->logw = 16;
     }
 }
 
@@ -289,10 +299,10 @@ static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse,
     // sub is read in the last pass, after A already holds intermediate values
     if (sub && (sub == A_dev || !same_or_disjoint(sub, A_dev, bytes))) return NTT_ERR_INVALID_ARGUMENT;
     if (plan_engine(plan, count) == NTT_ENGINE_FUSED) {
-        int eb = 0, stagger = 0;
-        bool lazy = false;
-        plan_fused_variant(plan, count, inverse, &eb, &lazy, &stagger);
-        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, eb, lazy, stagger, src, sub);
+        int stagger = 0;
+        FusedVariant v;
+        plan_fused_variant(plan, count, inverse, &v, &stagger);
+        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, v, stagger, src, sub);
     }
     TransformLaunch L = plan_launch(plan, A_dev, count, inverse);
     L.src = src;
@@ -322,6 +332,13 @@ int ntt_plan_set_fused_shape(ntt_plan* plan, int eb) {
     if (!plan) return NTT_ERR_INVALID_ARGUMENT;
     if (eb != 0 && eb != 6 && eb != 7) return NTT_ERR_CONFIG;
     plan->fused_eb = eb;
+    return NTT_OK;
+}
+
+int ntt_plan_set_fused_words(ntt_plan* plan, int log2_words) {
+    if (!plan) return NTT_ERR_INVALID_ARGUMENT;
+    if (log2_words != 0 && log2_words != 15 && log2_words != 16) return NTT_ERR_CONFIG;
+    plan->fused_logw = log2_words;
     return NTT_OK;
 }
 
@@ -367,13 +384,15 @@ int ntt_plan_set_fused_arith(ntt_plan* plan, int lazy) {
     return NTT_OK;
 }
 
-int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction, int* eb, int* lazy) {
+int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction, int* eb, int* lazy,
+                               int* log2_words) {
     if (!plan || (direction != NTT_FORWARD && direction != NTT_INVERSE)) return NTT_ERR_INVALID_ARGUMENT;
-    int e = 0, st = 0;
-    bool lz = false;
-    plan_fused_variant(plan, count, direction == NTT_INVERSE, &e, &lz, &st);
-    if (eb) *eb = e;
-    if (lazy) *lazy = lz ? 1 : 0;
+    int st = 0;
+    FusedVariant v;
+    plan_fused_variant(plan, count, direction == NTT_INVERSE, &v, &st);
+    if (eb) *eb = v.eb;
+    if (lazy) *lazy = v.lazy ? 1 : 0;
+    if (log2_words) *log2_words = v.logw;
     return NTT_OK;
 }
 

@@ -38,9 +38,10 @@
 #include "phase_kernels.hpp"
 
 namespace ntt {
-int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
+int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
+                 int logw = 16);
 int fused_prepare();
-uint32_t fused_items(int logN, size_t limbs);
+uint32_t fused_items(int logN, size_t limbs, int logw = 16);
 }  // namespace ntt
 
 using namespace ntt;
@@ -252,12 +253,22 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
             .push_back(i);
     size_t used = 0;                 // bytes of the slot's per-limb buffers, 16-byte aligned regions
     size_t nd = 0;
-    std::vector<std::tuple<int, bool, size_t, size_t, uint32_t, bool>> fl;   // n, inv, first, count, items, lazy-capable
+    struct FusedLaunch {
+        int n;
+        bool inv;
+        size_t first, count;
+        uint32_t items;
+        FusedVariant v;
+    };
+    std::vector<FusedLaunch> fl;
     for (auto& kv : groups) {
         const int n = std::get<0>(kv.first);
         const bool inv = std::get<1>(kv.first) == NTT_INVERSE;
         if (fused_supported(n)) {
             const size_t first = nd;
+            size_t tot = 0;
+            for (size_t i : kv.second) tot += reqs[i]->r.limbs;
+            const FusedVariant v = fused_variant_for(S->ctx, n, tot, inv, std::get<2>(kv.first) != 0);
             uint64_t items = 0;
             for (size_t i : kv.second) {
                 const Req& q = *reqs[i];
@@ -272,9 +283,9 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
                 j.item_begin = (uint32_t)items;
                 j.n = (uint16_t)n;
                 j.inverse = inv ? 1 : 0;
-                items += fused_items(n, q.r.limbs);
+                items += fused_items(n, q.r.limbs, v.logw);
             }
-            fl.emplace_back(n, inv, first, kv.second.size(), (uint32_t)items, std::get<2>(kv.first) != 0);
+            fl.push_back(FusedLaunch{n, inv, first, kv.second.size(), (uint32_t)items, v});
         } else {
             size_t T = 0;
             for (size_t i : kv.second) T += reqs[i]->r.limbs;
@@ -337,16 +348,10 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
     }
     for (auto& g : fl) {
         FusedParams p{};
-        p.jobs = s.d_desc + std::get<2>(g);
-        p.njobs = (uint32_t)std::get<3>(g);
-        p.total_items = std::get<4>(g);
-        const int n = std::get<0>(g);
-        size_t tot = 0;
-        for (size_t k = 0; k < std::get<3>(g); ++k) tot += s.h_desc[std::get<2>(g) + k].limbs;
-        int eb = 0;
-        bool lazy = false;
-        fused_variant_for(S->ctx, n, tot, std::get<1>(g), std::get<5>(g), &eb, &lazy);
-        if ((rc = launch_fused(n, std::get<1>(g), eb, lazy, p, s.stream, 0))) return rc;
+        p.jobs = s.d_desc + g.first;
+        p.njobs = (uint32_t)g.count;
+        p.total_items = g.items;
+        if ((rc = launch_fused(g.n, g.inv, g.v.eb, g.v.lazy, p, s.stream, 0, g.v.logw))) return rc;
         ++*fused;
     }
     return ck(hipEventRecord(s.done, s.stream), "hipEventRecord(slot)");

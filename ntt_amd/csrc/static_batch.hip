@@ -30,8 +30,9 @@
 #include "phase_kernels.hpp"
 
 namespace ntt {
-int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit);
-uint32_t fused_items(int logN, size_t limbs);
+int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
+                 int logw = 16);
+uint32_t fused_items(int logN, size_t limbs, int logw = 16);
 bool fused_supported(int logN);
 }  // namespace ntt
 

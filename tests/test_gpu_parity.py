@@ -277,11 +277,14 @@ def test_autotune_result_is_valid_and_exact(env, tmp_path):
 
 
 @pytest.mark.parametrize("logN", [10, 11, 12, 13, 14, 15, 16])
-@pytest.mark.parametrize("engine,eb,arith", [(2, 7, 1), (2, 7, 0), (2, 6, 1), (2, 6, 0), (1, 0, -1)])
-def test_engines_all_sizes(env, logN, engine, eb, arith):
-    """Fused limb kernel (both workgroup shapes; lazy and canonical forward
-    arithmetic) and two-pass kernels, bit-exact; limb counts that leave the
-    last fused work item partly empty."""
+@pytest.mark.parametrize("engine,eb,arith,logw", [(2, 7, 1, 16), (2, 7, 0, 16), (2, 6, 1, 16), (2, 6, 0, 16),
+                                                  (2, 7, 1, 15), (2, 7, 0, 15), (2, 6, 1, 15), (2, 6, 0, 15),
+                                                  (1, 0, -1, 16)])
+def test_engines_all_sizes(env, logN, engine, eb, arith, logw):
+    """Fused limb kernel (both window widths; lazy and canonical arithmetic in
+    both directions; 2^16 or, for logN <= 15, 2^15 words per workgroup) and
+    two-pass kernels, bit-exact; limb counts that leave the last fused work
+    item partly empty.  The inverse is checked on its own input too."""
     ntt_amd, gu, orc = env
     lpw = 1 << (16 - logN)
     for limbs in sorted({1, 3, lpw + 1}):
@@ -292,18 +295,25 @@ def test_engines_all_sizes(env, logN, engine, eb, arith):
             plan.set_engine(engine)
             plan.set_fused_shape(eb)
             plan.set_fused_arith(arith)
+            plan.set_fused_words(logw)
             assert plan.engine(1) == engine
             if engine == 2:
-                assert plan.fused_variant(1, 0) == (eb, arith == 1)
+                for direction in (0, 1):
+                    assert plan.fused_variant(1, direction) == (eb, arith == 1)
             x = gu.random_operand(logN, moduli, 1000 + limbs)
             A = gu.to_dev(x)
-            _print("case engine", engine, eb, arith, logN, limbs, domain)
+            _print("case engine", engine, eb, arith, logw, logN, limbs, domain)
             plan.forward(A)
             gu.sync()
             assert np.array_equal(gu.to_host(A).reshape(x.shape), orc.forward(x, logN, moduli, domain))
             plan.inverse(A)
             gu.sync()
             assert np.array_equal(gu.to_host(A).reshape(x.shape), x)
+            y = gu.random_operand(logN, moduli, 2000 + limbs)
+            B = gu.to_dev(y)
+            plan.inverse(B)
+            gu.sync()
+            assert np.array_equal(gu.to_host(B).reshape(y.shape), orc.inverse(y, logN, moduli, domain))
 
 
 def test_fused_many_operands_period(env):
