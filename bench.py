@@ -258,6 +258,254 @@ def bench_served(ntt_amd, requests: int = 20000, fractions=(0.25, 0.5, 0.75, 0.9
     return out
 
 
+def _timed(dev, stream, fn, reps):
+    e0, e1 = dev.Event(), dev.Event()
+    fn()
+    stream.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / reps
+
+
+def _pass_kernel_ms(dev, stream, plan, pool, count, cfg, lazy, twconv=False, steps=5):
+    """Per-pass kernel times of a two-pass launch (HIP events on the launch
+    stream), keyed by kernel symbol."""
+    mode = 1 if twconv else (2 if lazy else 0)
+    c = cfg.astuple()
+
+    def pk(ph, inv, col):   # the radix is a template argument for row passes, R 8-10, S >= 4
+        r, sm = c[ph][0], c[ph][1]
+        rf = r if (sm >= 4 and r in (8, 9, 10) and col == "false" and not twconv) else 0
+        return f"phase_kernel<{sm}, {inv}, {col}, {mode}, {rf}>"
+
+    kern = {pk(0, "false", "true"): lambda: plan.run_pass(pool, count, 0, 0, stream),
+            pk(1, "false", "false"): lambda: plan.run_pass(pool, count, 0, 1, stream),
+            pk(2, "true", "false"): lambda: plan.run_pass(pool, count, 1, 0, stream),
+            pk(3, "true", "true"): lambda: plan.run_pass(pool, count, 1, 1, stream)}
+    names = list(kern)
+    ev = [dev.Event() for _ in range(len(names) + 1)]
+    per = {n: [] for n in names}
+    for _ in range(steps):
+        ev[0].record(stream)
+        for i, n in enumerate(names):
+            kern[n]()
+            ev[i + 1].record(stream)
+        ev[-1].synchronize()
+        for i, n in enumerate(names):
+            per[n].append(ev[i].elapsed_ms(ev[i + 1]))
+    return {n: statistics.mean(v) for n, v in per.items()}
+
+
+def _roofline(kernel_ms: dict, bytes_per_launch: int, workload_tag: str, traffic_path: str, units: str):
+    dom = max(kernel_ms, key=kernel_ms.get)
+    achieved = bytes_per_launch / (kernel_ms[dom] * 1e-3) / 1e9
+    traffic = hbm_traffic(traffic_path, dom, workload_tag)
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_ratio": round(traffic / bytes_per_launch, 3) if traffic else None,
+            "kernel": dom, "kernel_ms": round(kernel_ms[dom], 4), "bytes_per_launch": bytes_per_launch,
+            "units_per_launch": units}
+
+
+def bench_c3(ntt_amd, dev, stream, traffic_path: str, tune: bool = True, count: int = 43, reps: int = 10):
+    """The larger single-transform shape: N = 2^17, 48 limbs (the 48 largest
+    primes q < 2^30 with q = 1 mod 2^18), Montgomery, a pool of `count`
+    operands (43 x 24 MiB = 1.03 GiB, beyond the last-level cache), forward
+    then inverse, roundtrip checked."""
+    import numpy as np
+    logN, limbs = 17, 48
+    N = 1 << logN
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    ctx = ntt_amd.Context(0)
+    cfg = None
+    if tune:
+        cfg = ntt_amd.TransformConfig.make(*ctx.autotune(logN, moduli, count=count, reps=2)["cfg"])
+    plan = ctx.plan(logN, moduli, ntt_amd.MONTGOMERY)
+    cfg = cfg or plan.config
+    words = limbs * N
+    pool = dev.DeviceBuffer(4 * words * count)
+    qd = dev.DeviceBuffer.from_numpy(np.array([q for q, _ in moduli], np.uint32))
+    dev.fill_uniform(pool, qd, limbs, logN, count, seed=0)
+    dev.synchronize()
+    before = pool.numpy(np.uint32, nbytes=4 * words)
+    fwd = _timed(dev, stream, lambda: plan.forward(pool, count, stream), reps)
+    inv = _timed(dev, stream, lambda: plan.inverse(pool, count, stream), reps)
+    ok = bool(np.array_equal(before, pool.numpy(np.uint32, nbytes=4 * words)))
+    bfly = 0.5 * N * logN * limbs * count
+    eng = {1: "two_pass", 2: "fused"}.get(plan.engine(count), "?")
+    kms = _pass_kernel_ms(dev, stream, plan, pool, count, cfg, all(q < (1 << 30) for q, _ in moduli))
+    step = fwd + inv
+    out = {"workload": f"N=2^17, 48 limbs (48 largest primes < 2^30, q = 1 mod 2^18), Montgomery, {count} operands "
+                       f"({4 * words * count / 2**30:.2f} GiB), forward then inverse, seed 0",
+           "engine": eng, "forward_ms": round(fwd, 4), "inverse_ms": round(inv, 4),
+           "gbfly_s": round(2 * bfly / (step * 1e-3) / 1e9, 1),
+           "hbm_frac": round(2 * 8 * words * count / (step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
+           "roofline": _roofline(kms, 8 * words * count, f"logN17x48_batch{count}", traffic_path,
+                                 f"{limbs * count} limbs of 2^17 words, 8 B per word per transform"),
+           "phase_config": cfg.astuple(), "roundtrip_ok": ok}
+    plan.close()
+    pool.free()
+    return out
+
+
+def bench_c1_plain(ntt_amd, dev, stream, traffic_path: str, logN: int = 16, limbs: int = 16, count: int = 256,
+                   reps: int = 5):
+    """The plain (normal-domain) variant at N = 2^16 x 16 limbs through the
+    drop-in launchers (src/v03_template_config.cu:549-603): normal-domain
+    twiddle tables converted on load.  Batched: the 256-operand pool as one
+    launch of 4096 limbs with the tables of each limb (as the reference's
+    launcher takes them); single transform: reference-style (restore copy +
+    transform) and kernel only, forward and inverse."""
+    import numpy as np
+    N = 1 << logN
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    t = ntt_amd.make_tables(logN, moduli, ntt_amd.PLAIN)
+    L = limbs * count
+    words = limbs * N
+    tw = dev.DeviceBuffer(4 * words * count)
+    itw = dev.DeviceBuffer(4 * words * count)
+    for k in range(count):
+        tw.upload(t.twiddles, offset_bytes=4 * words * k)
+        itw.upload(t.inv_twiddles, offset_bytes=4 * words * k)
+    q = dev.DeviceBuffer.from_numpy(np.tile(t.moduli, count))
+    dinv = dev.DeviceBuffer.from_numpy(np.tile(t.degree_inv, count))
+    pool = dev.DeviceBuffer(4 * words * count)
+    qd = dev.DeviceBuffer.from_numpy(np.array([m for m, _ in moduli], np.uint32))
+    dev.fill_uniform(pool, qd, limbs, logN, count, seed=0)
+    dev.synchronize()
+    before = pool.numpy(np.uint32, nbytes=4 * words)
+    f = lambda: ntt_amd.launch_ntt_plain(pool, tw, q, L, logN, None, stream)
+    i = lambda: ntt_amd.launch_intt_plain(pool, itw, q, dinv, L, logN, None, stream)
+    fwd = _timed(dev, stream, f, reps)
+    inv = _timed(dev, stream, i, reps)
+    ok = bool(np.array_equal(before, pool.numpy(np.uint32, nbytes=4 * words)))
+    bfly = 0.5 * N * logN * L
+    cfg = ntt_amd.default_launch_config(logN).astuple()
+    # single transform, reference-style: restore copy + transform (src/v03_template_config.cu timing loop)
+    one = dev.DeviceBuffer(4 * words)
+    src = dev.DeviceBuffer(4 * words)
+    src.copy_from(pool, 4 * words, stream)
+    single = {}
+    for name, fn in (("ntt", lambda: ntt_amd.launch_ntt_plain(one, tw, q, limbs, logN, None, stream)),
+                     ("intt", lambda: ntt_amd.launch_intt_plain(one, itw, q, dinv, limbs, logN, None, stream))):
+        single[f"single_{name}_ms_reference_style"] = round(
+            _timed(dev, stream, lambda: (one.copy_from(src, stream=stream), fn()), 30), 4)
+        single[f"single_{name}_ms_kernel_only"] = round(_timed(dev, stream, fn, 30), 4)
+    step = fwd + inv
+    out = {"workload": f"plain variant, N=2^{logN}, {limbs} limbs, {count} operands as one drop-in launch of {L} limbs "
+                       f"with per-limb normal-domain tables ({4 * words * count / 2**30:.2f} GiB operands), seed 0",
+           "forward_ms": round(fwd, 4), "inverse_ms": round(inv, 4),
+           "gbfly_s": round(2 * bfly / (step * 1e-3) / 1e9, 1),
+           "hbm_frac": round(2 * 8 * words * count / (step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "phase_config": cfg,
+           "kernels": {"forward": f"phase_kernel<{cfg[0][1]}, false, true, 1, 0> + phase_kernel<{cfg[1][1]}, false, false, 1, 0>",
+                       "inverse": f"phase_kernel<{cfg[2][1]}, true, false, 1, 0> + phase_kernel<{cfg[3][1]}, true, true, 1, 0>"},
+           "roofline": {"bound": "hbm", "achieved": round(8 * words * count / (max(fwd, inv) * 1e-3) / 1e9, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(8 * words * count / (max(fwd, inv) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                        "scope": "whole two-pass transform (both kernels) per direction",
+                        "bytes_per_launch": 8 * words * count},
+           **single, "roundtrip_ok": ok}
+    for b in (tw, itw, pool, one, src):
+        b.free()
+    return out
+
+
+def _cpu_model():
+    try:
+        return next(ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        return ""
+
+
+def cpu_jobs(jobs, seconds: float, threads: int):
+    """Oracle butterfly loops (src/v04_montgomery.cu:742-784 restated in C) over
+    the limbs of `jobs` = [(logN, moduli, direction, seed)], in order, limbs
+    of a job spread over `threads` host threads, until all are done or
+    `seconds` of compute have passed.  Input and table generation are not
+    timed.  Returns (limbs done, butterflies done, seconds, limbs total)."""
+    import ctypes
+    import concurrent.futures as cf
+
+    from oracle import oracle as orc
+
+    L = orc.lib()
+    P = ctypes.POINTER(ctypes.c_uint32)
+    tabs = {}
+    total = sum(len(m) for _, m, _, _ in jobs)
+
+    def one(w):
+        logN, q, key, direction, row = w
+        tw, itw, dinv = tabs[key]
+        a = row.copy()
+        if direction == 0:
+            L.oracle_cpu_ntt_butterfly(a.ctypes.data_as(P), logN, q, tw.ctypes.data_as(P))
+        else:
+            L.oracle_cpu_intt_butterfly(a.ctypes.data_as(P), logN, q, itw.ctypes.data_as(P), dinv)
+        return 0.5 * (1 << logN) * logN
+
+    done, bfly, busy = 0, 0.0, 0.0
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        for logN, moduli, direction, seed in jobs:
+            if busy >= seconds:
+                break
+            x = orc.fill_random_limbs(logN, moduli, seed)
+            work = []
+            for l, (q, g) in enumerate(moduli):
+                key = (logN, q, g)
+                if key not in tabs:
+                    rc, rt = orc.make_modulus_runtime(logN, q, g)
+                    tabs[key] = (orc.twiddle_table(logN, q, rt["psi"]), orc.twiddle_table(logN, q, rt["psi_inv"]),
+                                 rt["degree_inv"])
+                work.append((logN, q, key, direction, x[l]))
+            t0 = time.perf_counter()
+            bfly += sum(ex.map(one, work))
+            busy += time.perf_counter() - t0
+            done += len(work)
+    return done, bfly, busy, total
+
+
+def cpu_baselines_more(ntt_amd, seconds: float, threads: int):
+    """CPU baselines of the other workloads, each on a bounded sample:
+    the 17 x 48 shape, the uneven 256-job batch (its jobs in order) and a
+    500-request prefix of the served stream; full-workload times are
+    extrapolated from the sample rate."""
+    from ntt_amd import workloads
+    model = _cpu_model()
+    out = {}
+    m48 = [tuple(m) for m in ntt_amd.generate_moduli(17, 48)]
+    c3_jobs = [(17, m48, d, k) for d in (0, 1) for k in range(43)]
+    d, b, t, n = cpu_jobs(c3_jobs, seconds, threads)
+    out["c3"] = {"value": round(b / t / 1e9, 3), "unit": "Gbutterfly/s", "cores": threads, "kind": "port",
+                 "extrapolated_ms": round(2 * 43 * 48 * 0.5 * (1 << 17) * 17 / (b / t) * 1e3, 1),
+                 "sample": f"{d} of {n} limbs of the 17 x 48 workload (43 operands forward, then 43 inverse, "
+                           f"seed k per operand), {t:.1f} s, {threads} thread(s) on {model}"}
+    wl = workloads.batched_uneven(0)
+    jobs = [(j.logN, workloads.moduli_for(j.logN, j.limbs), j.direction, k) for k, j in enumerate(wl)]
+    d, b, t, n = cpu_jobs(jobs, seconds, threads)
+    total = sum(j.butterflies for j in wl)
+    out["mixed"] = {"value": round(b / t / 1e9, 3), "unit": "Gbutterfly/s", "cores": threads, "kind": "port",
+                    "extrapolated_ms": round(total / (b / t) * 1e3, 1),
+                    "sample": f"first {d} of {n} limbs of the 256-job uneven workload (seed 0), {t:.1f} s, "
+                              f"{threads} thread(s) on {model}; full-workload time extrapolated from the rate"}
+    reqs = workloads.serving_stream(0, 500, rate_per_s=1.0)
+    jobs = [(r.job.logN, workloads.moduli_for(r.job.logN, r.job.limbs), r.job.direction, k)
+            for k, r in enumerate(reqs)]
+    d, b, t, n = cpu_jobs(jobs, 4 * seconds, threads)
+    rate = b / t
+    req_s = 500 / (t * n / max(d, 1))
+    out["served"] = {"value": round(rate / 1e9, 3), "unit": "Gbutterfly/s", "cores": threads, "kind": "port",
+                     "requests_per_s": round(req_s, 1),
+                     "extrapolated_20k_s": round(20000 / req_s, 1),
+                     "sample": f"{d} of {n} limbs of a 500-request prefix of the served stream (seed 0), "
+                               f"{t:.1f} s, {threads} thread(s) on {model}; 20k-request time extrapolated"}
+    return out
+
+
 def cpu_baseline(logN: int, moduli, seconds: float, threads: int = 1):
     """Oracle butterfly loops on host cores over a bounded sample: forward +
     inverse of whole limbs of the workload (seed 0 inputs), until `seconds`."""
@@ -484,11 +732,20 @@ def main():
         ev_b.record(stream)
         ev_b.synchronize()
         ki_ms = ev_a.elapsed_ms(ev_b) / iters
+        # reference-style INTT: restore copy + inverse (src/v04_montgomery.cu:1037-1062)
+        ev_a.record(stream)
+        for _ in range(iters):
+            one.copy_from(src, stream=stream)
+            plan.inverse(one, 1, stream)
+        ev_b.record(stream)
+        ev_b.synchronize()
+        refi_ms = ev_a.elapsed_ms(ev_b) / iters
         extras.update({
             "single_ntt_ms_reference_style": round(ref_ms, 4),
             "single_ntt_gbfly_s_reference_style": round(bfly_per_transform / (ref_ms * 1e-3) / 1e9, 1),
             "single_ntt_ms_kernel_only": round(k_ms, 4),
             "single_intt_ms_kernel_only": round(ki_ms, 4),
+            "single_intt_ms_reference_style": round(refi_ms, 4),
             "vs_rtx4090_v04_ntt_reference_style": round(bfly_per_transform / (ref_ms * 1e-3) / 1e9
                                                         / RTX4090_V04_NTT_GBFLY, 3),
         })
@@ -533,6 +790,13 @@ def main():
         extras["word64"] = bench_word64(ntt_amd, dev, stream, logN, limbs)
         log("extras: " + json.dumps(extras))
 
+    c3 = c1 = None
+    if not args.no_extras:
+        c3 = bench_c3(ntt_amd, dev, stream, args.traffic, tune=not args.no_tune)
+        log("c3: " + json.dumps(c3))
+        c1 = bench_c1_plain(ntt_amd, dev, stream, args.traffic)
+        log("c1_plain: " + json.dumps(c1))
+
     mixed = None
     if not args.no_mixed:
         mixed = bench_mixed(ntt_amd, dev, stream, tune=not args.no_tune)
@@ -543,7 +807,7 @@ def main():
         served = bench_served(ntt_amd, requests=args.served_requests, tune=not args.no_tune)
         log("served: " + json.dumps({k: v for k, v in served.items() if k != "sweep"}))
 
-    cpu = cpu1 = None
+    cpu = cpu1 = cpu_more = None
     if not args.no_cpu:
         # limbs are independent: spread over the host threads this job may use
         # (at most 16, the CPU share of one GPU on the box)
@@ -552,6 +816,8 @@ def main():
         log(f"cpu baseline ({threads} threads): {cpu}")
         cpu1 = cpu_baseline(logN, moduli, args.cpu_seconds / 2, threads=1)
         log(f"cpu baseline (1 thread): {cpu1}")
+        cpu_more = cpu_baselines_more(ntt_amd, args.cpu_seconds / 2, threads)
+        log(f"cpu baselines (other workloads): {cpu_more}")
 
     out = {
         "metric": METRIC,
@@ -584,6 +850,9 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "cpu_baseline_1core": cpu1,
+        "cpu_baseline_other": cpu_more,
+        "c3": c3,
+        "c1_plain": c1,
         "mixed": mixed,
         "served": served,
         "compute_roof": compute_roof,
