@@ -259,6 +259,29 @@ int ntt_plan_inverse_to(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_d
  * ntt_plan_forward_mod_down, e.g. P^-1 mod q_l; NULL releases them.
  * Synchronous; the first call allocates `limbs` words of device memory. */
 int ntt_plan_set_mod_down_factors(ntt_plan* plan, const uint32_t* factors);
+/* The sub operand of the mod-down form restricted to limbs
+ * [sub_start, sub_end) of the plan, as Cheddar's src2 limb range
+ * (src/NTT_cheddar.cu:566-590): `sub` then holds sub_end - sub_start limbs
+ * per operand, limb l of the plan reads sub limb l - sub_start, and limbs
+ * outside the range compute dst = (0 - NTT) * f.  sub_factors (host,
+ * sub_end - sub_start values, or NULL) multiplies each sub limb first, as
+ * Cheddar's src2_padding (579-584).  (0, limbs, NULL) is the default: every
+ * limb, no multiplier. */
+int ntt_plan_set_mod_down_sub(ntt_plan* plan, uint32_t sub_start, uint32_t sub_end, const uint32_t* sub_factors);
+/* NTTForModUp with a skipped limb range (src/NTT_cheddar.cu:815-887): the
+ * forward transform of every limb of each of `count` operands except limbs
+ * [skip_start, skip_end), in one launch; dst limbs in that range are left
+ * untouched.  src NULL or equal to dst: in place; otherwise src has dst's
+ * layout. */
+int ntt_plan_forward_skip(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, uint32_t skip_start,
+                          uint32_t skip_end, size_t count, void* stream);
+/* INTTForModDown over a limb subset of its source (src/NTT_cheddar.cu:974-1049,
+ * the np_non_intt case): dst (count operands of the plan's limbs) =
+ * INTT of limbs [src_first, src_first + limbs) of `count` source operands of
+ * src_limbs limbs each, times the plan's epilogue constants; src must not
+ * overlap dst. */
+int ntt_plan_inverse_from(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, uint32_t src_limbs,
+                          uint32_t src_first, size_t count, void* stream);
 /* dst = (sub - NTT(src)) * f_l, limb by limb, in the forward transform's
  * output order and domain (sub holds canonical residues in that order and
  * domain): the NTT and the subtract-and-scale of Cheddar's NTTForModDown

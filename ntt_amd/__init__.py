@@ -237,6 +237,9 @@ _PROTOS = {
     "ntt_plan_inverse_to": (_I, [_P, _P, _P, _SZ, _P]),
     "ntt_plan_set_mod_down_factors": (_I, [_P, _PU32]),
     "ntt_plan_forward_mod_down": (_I, [_P, _P, _P, _P, _SZ, _P]),
+    "ntt_plan_set_mod_down_sub": (_I, [_P, _U32, _U32, _PU32]),
+    "ntt_plan_forward_skip": (_I, [_P, _P, _P, _U32, _U32, _SZ, _P]),
+    "ntt_plan_inverse_from": (_I, [_P, _P, _P, _U32, _U32, _SZ, _P]),
     "ntt_plan_run_pass": (_I, [_P, _P, _SZ, _I, _I, _P]),
     "ntt_plan_set_engine": (_I, [_P, _I]),
     "ntt_plan_set_fused_shape": (_I, [_P, _I]),
@@ -750,6 +753,23 @@ class Plan:
         """dst = (sub - NTT(src)) * f per limb, in the NTT output's order and domain."""
         _check(_lib.ntt_plan_forward_mod_down(self.handle, _addr(dst), _addr(src), _addr(sub), count,
                                               _stream(stream)), "ntt_plan_forward_mod_down")
+
+    def set_mod_down_sub(self, sub_start: int, sub_end: int, factors=None):
+        """Mod-down sub operand over limbs [sub_start, sub_end) with optional multipliers."""
+        arr = None
+        if factors is not None:
+            arr = np.ascontiguousarray(np.array([int(f) for f in factors], dtype=np.uint32))
+        _check(_lib.ntt_plan_set_mod_down_sub(self.handle, sub_start, sub_end,
+                                              arr.ctypes.data_as(_PU32) if arr is not None else None),
+               "ntt_plan_set_mod_down_sub")
+
+    def forward_skip(self, dst, src, skip_start: int, skip_end: int, count: int = 1, stream=None):
+        _check(_lib.ntt_plan_forward_skip(self.handle, _addr(dst), _addr(src), skip_start, skip_end, count,
+                                          _stream(stream)), "ntt_plan_forward_skip")
+
+    def inverse_from(self, dst, src, src_limbs: int, src_first: int, count: int = 1, stream=None):
+        _check(_lib.ntt_plan_inverse_from(self.handle, _addr(dst), _addr(src), src_limbs, src_first, count,
+                                          _stream(stream)), "ntt_plan_inverse_from")
 
     def set_engine(self, engine: int):
         _check(_lib.ntt_plan_set_engine(self.handle, engine), "ntt_plan_set_engine")

@@ -238,7 +238,7 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         for (size_t i : kv.second) {
             const ntt_batch_job& j = b->jobs[i];
             const ntt_plan* P = b->job_plan[i];
-            FusedJob fj{};
+            FusedJob fj = fused_job_plain(j.limbs);
             fj.data = j.data;
             fj.src = j.data;
             fj.tw = g.inverse ? P->d_tw_inv : P->d_tw_fwd;

@@ -32,6 +32,15 @@ struct TwiddleCache {
 
 struct LimbConst;
 
+// Which limbs of `count` contiguous operands a plan launch transforms and
+// where it reads them: limbs [skip_start, skip_start + skip_len) of every
+// operand are skipped; the input operands have src_period limbs (0: the
+// plan's) and the plan's limb p is read from their limb src_first + p.
+struct LimbMap {
+    uint32_t skip_start = 0, skip_len = 0;
+    uint32_t src_period = 0, src_first = 0;
+};
+
 // Autotuner result for one (logN, words per launch) bucket.
 struct TunedEntry {
     ntt_transform_config cfg;
@@ -85,6 +94,11 @@ struct ntt_plan {
     bool prologue = false;                         // forward input multiplied by per-limb constants
     uint32_t* d_post = nullptr;                    // mod-down multipliers to_mont(f_l), own allocation
     bool twiddle_ot = false;                       // fused engine: on-the-fly twiddles (NTT_TWIDDLE_OT)
+    // mod-down sub operand: limbs [sub_start, sub_end) only (when sub_range),
+    // each multiplied by d_subf[l - sub_start] = to_mont(factor) when non-null
+    bool sub_range = false;
+    uint32_t sub_start = 0, sub_end = 0;
+    uint32_t* d_subf = nullptr;                    // own allocation
 };
 
 namespace ntt {
@@ -98,7 +112,8 @@ int log2_bucket(size_t v);
 // Engine a launch of `count` operands runs on (resolves NTT_ENGINE_AUTO).
 int plan_engine(const ntt_plan* P, size_t count);
 int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
-                      const FusedVariant& v, int stagger, const uint32_t* src = nullptr, const uint32_t* sub = nullptr);
+                      const FusedVariant& v, int stagger, const uint32_t* src = nullptr, const uint32_t* sub = nullptr,
+                      const LimbMap& map = LimbMap());
 // The fused kernel variant a run of `count` operands uses in `inverse` direction.
 void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, FusedVariant* v, int* stagger);
 // Fused variant for a launch of `total_limbs` limbs of size 2^logN from the

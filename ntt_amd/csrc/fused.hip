@@ -123,14 +123,25 @@ int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipSt
 // from `src` (null: in place) and written to A; with `sub` non-null, the
 // forward mod-down form (the plan's per-limb multipliers).
 int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
-                      const FusedVariant& v, int stagger, const uint32_t* src, const uint32_t* sub) {
+                      const FusedVariant& v, int stagger, const uint32_t* src, const uint32_t* sub,
+                      const LimbMap& map) {
     if (!fused_supported(P->logN)) return NTT_ERR_CONFIG;
     if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(sub)) & 15)
         return NTT_ERR_INVALID_ARGUMENT;
     if (sub && (inverse || !P->d_post)) return NTT_ERR_INVALID_ARGUMENT;
-    const size_t limbs = P->limbs * count;
+    const size_t limbs = (P->limbs - map.skip_len) * count;
     if (limbs > 0xffffffffu) return NTT_ERR_INVALID_ARGUMENT;
     FusedParams p{};
+    p.job0 = fused_job_plain((uint32_t)P->limbs);
+    p.job0.skip_start = map.skip_start;
+    p.job0.skip_len = map.skip_len;
+    if (map.src_period) p.job0.src_period = map.src_period;
+    p.job0.src_first = map.src_first;
+    if (sub && P->sub_range) {
+        p.job0.sub_start = P->sub_start;
+        p.job0.sub_end = P->sub_end;
+    }
+    p.job0.subf = sub ? P->d_subf : nullptr;
     p.job0.data = A;
     p.job0.src = src ? src : A;
     p.job0.sub = sub;
@@ -138,7 +149,6 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.job0.tw = inverse ? P->d_tw_inv : P->d_tw_fwd;
     p.job0.lc = inverse ? P->d_lc_inv : P->d_lc_fwd;
     p.job0.limbs = (uint32_t)limbs;
-    p.job0.period = (uint32_t)P->limbs;
     p.job0.item_begin = 0;
     p.job0.n = (uint16_t)P->logN;
     p.job0.inverse = inverse ? 1 : 0;

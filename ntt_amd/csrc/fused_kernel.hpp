@@ -252,9 +252,16 @@ __device__ __forceinline__ FusedJob uniform_job(const FusedJob& j) {
     u.post = uniform_ptr(j.post);
     u.tw = uniform_ptr(j.tw);
     u.lc = uniform_ptr(j.lc);
+    u.subf = uniform_ptr(j.subf);
     u.limbs = __builtin_amdgcn_readfirstlane(j.limbs);
     u.period = __builtin_amdgcn_readfirstlane(j.period);
     u.item_begin = __builtin_amdgcn_readfirstlane(j.item_begin);
+    u.skip_start = __builtin_amdgcn_readfirstlane(j.skip_start);
+    u.skip_len = __builtin_amdgcn_readfirstlane(j.skip_len);
+    u.src_period = __builtin_amdgcn_readfirstlane(j.src_period);
+    u.src_first = __builtin_amdgcn_readfirstlane(j.src_first);
+    u.sub_start = __builtin_amdgcn_readfirstlane(j.sub_start);
+    u.sub_end = __builtin_amdgcn_readfirstlane(j.sub_end);
     u.n = j.n;
     u.inverse = j.inverse;
     return u;
@@ -277,8 +284,12 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     const uint32_t limb = first_limb + slot;
     const bool valid = limb < job.limbs;
     const uint32_t lsafe = valid ? limb : first_limb;
-    const uint32_t per = lsafe % job.period;
-    gcu32* In = (gcu32*)(job.src + ((size_t)lsafe << n));
+    // job limb -> (operand, plan limb), skipping [skip_start, skip_start + skip_len)
+    const uint32_t vper = job.period - job.skip_len;
+    const uint32_t op = lsafe / vper, pv = lsafe - op * vper;
+    const uint32_t per = pv + (pv >= job.skip_start ? job.skip_len : 0);
+    const size_t dlimb = (size_t)op * job.period + per;
+    gcu32* In = (gcu32*)(job.src + (((size_t)op * job.src_period + job.src_first + per) << n));
     gcu32* tw = (gcu32*)job.tw[per];
     LimbConst c = job.lc[per];
     if constexpr (G_BITS >= 6) {   // one limb per wave: the limb's table and constants are wave-uniform
@@ -343,29 +354,42 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
             for (int x = 0; x < E; ++x) v[x] = canon_4q(v[x], c.q);
         }
         if constexpr (MD) {
-            // mod-down: (sub - NTT) * post, read at the positions stored below
+            // mod-down: (sub - NTT) * post, read at the positions stored below;
+            // limbs outside [sub_start, sub_end) subtract from 0
             uint32_t f = job.post[per];
             if constexpr (G_BITS >= 6) f = __builtin_amdgcn_readfirstlane(f);
-            gcu32* Sb = (gcu32*)(job.sub + ((size_t)lsafe << n)) + ubase<LOL, EB>(opaque(u));
-            if constexpr (LOL == 0) {
-                static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
-                    constexpr int x4 = decltype(X4)::value;
-                    const u32x4 s4 = ((gcu4*)Sb)[x4];
-                    const uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
-                    static_for<4>([&](auto K4) __attribute__((always_inline)) {
-                        constexpr int r = REG<n, INV, NR - 1, 4 * x4 + decltype(K4)::value, EB>;
-                        v[r] = mont_mul(sub_mod(sv[decltype(K4)::value], v[r], c.q), f, c.q, c.q_inv);
+            const bool has_sub = per >= job.sub_start && per < job.sub_end;
+            if (has_sub) {
+                const uint32_t sw = job.sub_end - job.sub_start, sl = per - job.sub_start;
+                uint32_t g = job.subf ? job.subf[sl] : 0u;
+                if constexpr (G_BITS >= 6) g = __builtin_amdgcn_readfirstlane(g);
+                gcu32* Sb = (gcu32*)(job.sub + (((size_t)op * sw + sl) << n)) + ubase<LOL, EB>(opaque(u));
+                auto one = [&](uint32_t s) __attribute__((always_inline)) {
+                    return job.subf ? mont_mul(s, g, c.q, c.q_inv) : s;
+                };
+                if constexpr (LOL == 0) {
+                    static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
+                        constexpr int x4 = decltype(X4)::value;
+                        const u32x4 s4 = ((gcu4*)Sb)[x4];
+                        const uint32_t sv[4] = {s4.x, s4.y, s4.z, s4.w};
+                        static_for<4>([&](auto K4) __attribute__((always_inline)) {
+                            constexpr int r = REG<n, INV, NR - 1, 4 * x4 + decltype(K4)::value, EB>;
+                            v[r] = mont_mul(sub_mod(one(sv[decltype(K4)::value]), v[r], c.q), f, c.q, c.q_inv);
+                        });
                     });
-                });
+                } else {
+                    static_for<E>([&](auto X) __attribute__((always_inline)) {
+                        constexpr int x = decltype(X)::value;
+                        constexpr int r = REG<n, INV, NR - 1, x, EB>;
+                        v[r] = mont_mul(sub_mod(one(Sb[x << LOL]), v[r], c.q), f, c.q, c.q_inv);
+                    });
+                }
             } else {
-                static_for<E>([&](auto X) __attribute__((always_inline)) {
-                    constexpr int x = decltype(X)::value;
-                    constexpr int r = REG<n, INV, NR - 1, x, EB>;
-                    v[r] = mont_mul(sub_mod(Sb[x << LOL], v[r], c.q), f, c.q, c.q_inv);
-                });
+#pragma unroll
+                for (int x = 0; x < E; ++x) v[x] = mont_mul(sub_mod(0u, v[x], c.q), f, c.q, c.q_inv);
             }
         }
-        gu32* A = (gu32*)(job.data + ((size_t)lsafe << n));
+        gu32* A = (gu32*)(job.data + (dlimb << n));
         if (LOL == 0) {
             gu4* dst = (gu4*)(A + ubase<LOL, EB>(opaque(u)));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {

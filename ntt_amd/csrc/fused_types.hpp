@@ -43,11 +43,17 @@ struct LimbConst {
     uint32_t flags;    // bit 0: apply scale
 };
 
-// A run of `limbs` limbs of size 2^n stored back to back at `data`; limb l
-// uses table tw[l % period] and constants lc[l % period].  The input is read
-// from `src` (same layout; equal to `data` in place).  The mod-down kernels
-// also read `sub` (same layout as `data`) and the per-limb Montgomery
-// multipliers post[l % period], and store (sub - transform) * post.
+// Operands of `period` limbs of size 2^n stored back to back at `data`.  The
+// job transforms `limbs` limbs: limb v of the job is limb p of operand
+// o = v / (period - skip_len), where p runs over [0, period) except the
+// skipped range [skip_start, skip_start + skip_len).  Limb p uses table
+// tw[p] and constants lc[p], is read from src + (o * src_period + src_first
+// + p) * 2^n (src == data, src_period == period, src_first == 0 in place)
+// and written to data + (o * period + p) * 2^n.
+// The mod-down kernels also read `sub`, which holds, per operand,
+// sub_end - sub_start limbs for plan limbs [sub_start, sub_end), optionally
+// multiplied by the Montgomery multipliers subf[p - sub_start]; limbs
+// outside the range read 0.  They store (sub - transform) * post[p].
 struct FusedJob {
     uint32_t* data;
     const uint32_t* src;
@@ -55,12 +61,26 @@ struct FusedJob {
     const uint32_t* post;
     const uint32_t* const* tw;
     const LimbConst* lc;
+    const uint32_t* subf;
     uint32_t limbs;
     uint32_t period;
     uint32_t item_begin;   // first work item of this job (prefix sum)
     uint16_t n;
     uint16_t inverse;      // descriptive: the kernel's direction is its template parameter
+    uint32_t skip_start, skip_len;
+    uint32_t src_period, src_first;
+    uint32_t sub_start, sub_end;
 };
+
+// A job over `limbs` whole operands' limbs of `period` limbs each, in place,
+// no skipped limbs, full mod-down range.
+inline FusedJob fused_job_plain(uint32_t period) {
+    FusedJob j{};
+    j.period = period;
+    j.src_period = period;
+    j.sub_end = period;
+    return j;
+}
 
 struct FusedParams {
     FusedJob job0;            // used when jobs == nullptr

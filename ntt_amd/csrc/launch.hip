@@ -89,7 +89,7 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, int mode,
     p.R = R;
     p.W = W;
     p.tiles = 1u << (p.logN - R - W);
-    const size_t blocks = (size_t)p.tiles * p.limbs * p.count;
+    const size_t blocks = (size_t)p.tiles * (p.limbs - p.skip_len) * p.count;
     if (blocks == 0 || blocks > 0x7fffffffu) return NTT_ERR_INVALID_ARGUMENT;
     p.xcd_remap = (base.xcd_remap && blocks % 8 == 0) ? 1 : 0;
     const unsigned threads = 1u << (R - S + W);
@@ -111,11 +111,13 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, int mode,
 int launch_transform(const TransformLaunch& L, hipStream_t stream) {
     if (L.A_ptrs || L.chunk == 0 || L.chunk >= L.count) return launch_transform_passes(L, stream, 3);
     const size_t words = L.limbs << L.logN;
+    const size_t src_words = (L.src_period ? (size_t)L.src_period : L.limbs) << L.logN;
+    const size_t sub_words = (L.sub_range ? (size_t)(L.sub_end - L.sub_start) : L.limbs) << L.logN;
     for (size_t first = 0; first < L.count; first += L.chunk) {
         TransformLaunch c = L;
         c.A = L.A + first * words;
-        if (L.src) c.src = L.src + first * words;
-        if (L.sub) c.sub = L.sub + first * words;
+        if (L.src) c.src = L.src + first * src_words;
+        if (L.sub) c.sub = L.sub + first * sub_words;
         c.count = L.count - first < L.chunk ? L.count - first : L.chunk;
         int rc = launch_transform_passes(c, stream, 3);
         if (rc) return rc;
@@ -135,6 +137,9 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
     // out-of-place and mod-down launches run whole transforms on contiguous operands
     if ((L.src || L.sub) && (L.A_ptrs || pass_mask != 3)) return NTT_ERR_INVALID_ARGUMENT;
     if (L.sub && (L.inverse || !L.post)) return NTT_ERR_INVALID_ARGUMENT;
+    if (L.A_ptrs && (L.skip_len || L.src_period || L.src_first || L.sub_range)) return NTT_ERR_INVALID_ARGUMENT;
+    if ((size_t)L.skip_start + L.skip_len > L.limbs || L.skip_len >= L.limbs) return NTT_ERR_INVALID_ARGUMENT;
+    if (L.sub_range && (L.sub_start > L.sub_end || L.sub_end > L.limbs)) return NTT_ERR_INVALID_ARGUMENT;
     int rc = validate_config(L.logN, &L.cfg, device_max_lds());
     if (rc) return rc;
 
@@ -149,6 +154,13 @@ int launch_transform_passes(const TransformLaunch& L, hipStream_t stream, int pa
     p.limbs = (uint32_t)L.limbs;
     p.count = (uint32_t)L.count;
     p.xcd_remap = L.xcd_remap;
+    p.skip_start = L.skip_start;
+    p.skip_len = L.skip_len;
+    p.src_period = L.src_period ? L.src_period : (uint32_t)L.limbs;
+    p.src_first = L.src_first;
+    p.sub_start = L.sub_range ? L.sub_start : 0;
+    p.sub_end = L.sub_range ? L.sub_end : (uint32_t)L.limbs;
+    p.subf = L.subf;
 
     // The scale multiplies on the first load of a forward transform and on
     // the last store of an inverse one, in whichever pass that is.

@@ -33,6 +33,16 @@ struct TransformLaunch {
     // intermediate between the passes of one chunk can stay in the caches.
     size_t chunk;
     ntt_transform_config cfg;
+    // Limb mapping (contiguous operands only; see PhaseParams): limbs
+    // [skip_start, skip_start + skip_len) of each operand are not
+    // transformed; src operand stride src_period limbs (0: limbs) starting
+    // at limb src_first; mod-down sub range [sub_start, sub_end) when
+    // sub_range (else every limb) with optional multipliers subf.
+    uint32_t skip_start, skip_len;
+    uint32_t src_period, src_first;
+    bool sub_range;
+    uint32_t sub_start, sub_end;
+    const uint32_t* subf;
 };
 
 int launch_transform(const TransformLaunch& L, hipStream_t stream);

@@ -62,6 +62,16 @@ struct PhaseParams {
     uint32_t count;           // operands
     int xcd_remap;            // 1: group workgroups of the same limb index on one XCD
     int final_pass;           // 1: this pass ends the transform (outputs made canonical)
+    // Limb mapping (contiguous operands only): the grid covers limbs - skip_len
+    // limbs per operand, plan limbs [skip_start, skip_start + skip_len) are
+    // skipped; the input of plan limb p of operand o is src (or A) +
+    // (o * src_period + src_first + p) * N.  Mod-down: sub holds sub_end -
+    // sub_start limbs per operand for plan limbs [sub_start, sub_end), each
+    // optionally multiplied by subf[p - sub_start]; other limbs subtract from 0.
+    uint32_t skip_start, skip_len;
+    uint32_t src_period, src_first;
+    uint32_t sub_start, sub_end;
+    const uint32_t* subf;
 };
 
 // Arithmetic variants of the pass kernels.
@@ -132,11 +142,12 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     const uint32_t tile = b % p.tiles;
     const uint32_t rest = b / p.tiles;
     const uint32_t op = rest % p.count;
-    const uint32_t li = rest / p.count;
+    const uint32_t lv = rest / p.count;
+    const uint32_t li = lv + (lv >= p.skip_start ? p.skip_len : 0);   // plan limb
     const size_t N = (size_t)1 << n;
     const size_t off = ((size_t)op * p.limbs + li) * N;
     gu32* A = (gu32*)(p.A_ptrs ? p.A_ptrs[li] : p.A + off);
-    gcu32* In = p.src ? (gcu32*)(p.src + off) : A;
+    gcu32* In = p.src ? (gcu32*)(p.src + ((size_t)op * p.src_period + p.src_first + li) * N) : A;
     gcu32* tw = (gcu32*)(p.tw_ptrs ? p.tw_ptrs[li] : p.tw + (size_t)li * N);
 
     const uint32_t q = p.moduli[li];
@@ -261,7 +272,18 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
                 // (row passes only: the forward transform ends in its row pass)
                 const uint32_t f = p.post[li];
                 uint32_t s[E];
-                gload((gcu32*)(p.sub + off), lo, s);
+                if (li >= p.sub_start && li < p.sub_end) {
+                    const uint32_t sl = li - p.sub_start;
+                    gload((gcu32*)(p.sub + ((size_t)op * (p.sub_end - p.sub_start) + sl) * N), lo, s);
+                    if (p.subf) {
+                        const uint32_t g = p.subf[sl];
+#pragma unroll
+                        for (int x = 0; x < E; ++x) s[x] = mont_mul(s[x], g, q, q_inv);
+                    }
+                } else {
+#pragma unroll
+                    for (int x = 0; x < E; ++x) s[x] = 0;
+                }
 #pragma unroll
                 for (int x = 0; x < E; ++x) v[x] = mont_mul(sub_mod(s[x], v[x], q), f, q, q_inv);
             }

@@ -271,6 +271,7 @@ int ntt_plan_destroy(ntt_plan* plan) {
     if (!plan) return NTT_ERR_INVALID_ARGUMENT;
     if (plan->d_const) (void)hipFree(plan->d_const);
     if (plan->d_post) (void)hipFree(plan->d_post);
+    if (plan->d_subf) (void)hipFree(plan->d_subf);
     delete plan;
     return NTT_OK;
 }
@@ -289,25 +290,47 @@ static bool same_or_disjoint(const void* a, const void* b, size_t bytes) {
 
 // `count` operands: in place (src null), out of place from src, or (forward,
 // sub non-null) the mod-down form A = (sub - NTT(src or A)) * f.
+// Two device ranges overlap.
+static bool overlap(const void* a, size_t abytes, const void* b, size_t bbytes) {
+    const uintptr_t x = reinterpret_cast<uintptr_t>(a), y = reinterpret_cast<uintptr_t>(b);
+    return x < y + bbytes && y < x + abytes;
+}
+
 static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse, void* stream,
-                    const uint32_t* src = nullptr, const uint32_t* sub = nullptr) {
+                    const uint32_t* src = nullptr, const uint32_t* sub = nullptr, const LimbMap& map = LimbMap()) {
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
-    const size_t bytes = (plan->limbs << plan->logN) * count * sizeof(uint32_t);
-    if (src == A_dev) src = nullptr;
-    if (src && !same_or_disjoint(src, A_dev, bytes)) return NTT_ERR_INVALID_ARGUMENT;
+    const size_t N = (size_t)1 << plan->logN;
+    const size_t bytes = plan->limbs * N * count * sizeof(uint32_t);
+    if (src == A_dev && !map.src_period && !map.src_first) src = nullptr;
+    if (map.src_period || map.src_first) {
+        // a limb range of wider input operands: a separate buffer
+        if (!src || overlap(src, (size_t)map.src_period * N * count * sizeof(uint32_t), A_dev, bytes))
+            return NTT_ERR_INVALID_ARGUMENT;
+    } else if (src && !same_or_disjoint(src, A_dev, bytes)) {
+        return NTT_ERR_INVALID_ARGUMENT;
+    }
     if (sub && (inverse || !plan->d_post)) return NTT_ERR_INVALID_ARGUMENT;
     // sub is read in the last pass, after A already holds intermediate values
-    if (sub && (sub == A_dev || !same_or_disjoint(sub, A_dev, bytes))) return NTT_ERR_INVALID_ARGUMENT;
+    const size_t sub_bytes = (plan->sub_range ? plan->sub_end - plan->sub_start : plan->limbs) * N * count * 4;
+    if (sub && (sub == A_dev || overlap(sub, sub_bytes, A_dev, bytes))) return NTT_ERR_INVALID_ARGUMENT;
     if (plan_engine(plan, count) == NTT_ENGINE_FUSED) {
         int stagger = 0;
         FusedVariant v;
         plan_fused_variant(plan, count, inverse, &v, &stagger);
-        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, v, stagger, src, sub);
+        return plan_launch_fused(plan, A_dev, count, inverse, (hipStream_t)stream, v, stagger, src, sub, map);
     }
     TransformLaunch L = plan_launch(plan, A_dev, count, inverse);
     L.src = src;
     L.sub = sub;
     L.post = sub ? plan->d_post : nullptr;
+    L.skip_start = map.skip_start;
+    L.skip_len = map.skip_len;
+    L.src_period = map.src_period;
+    L.src_first = map.src_first;
+    L.sub_range = sub && plan->sub_range;
+    L.sub_start = plan->sub_start;
+    L.sub_end = plan->sub_end;
+    L.subf = sub ? plan->d_subf : nullptr;
     return launch_transform(L, (hipStream_t)stream);
 }
 
@@ -447,6 +470,50 @@ int ntt_plan_forward_mod_down(ntt_plan* plan, uint32_t* dst_dev, const uint32_t*
                               size_t count, void* stream) {
     if (!src_dev || !sub_dev) return NTT_ERR_INVALID_ARGUMENT;
     return plan_run(plan, dst_dev, count, false, stream, src_dev, sub_dev);
+}
+
+int ntt_plan_set_mod_down_sub(ntt_plan* plan, uint32_t sub_start, uint32_t sub_end, const uint32_t* sub_factors) {
+    if (!plan || sub_start > sub_end || sub_end > plan->limbs) return NTT_ERR_INVALID_ARGUMENT;
+    const uint32_t w = sub_end - sub_start;
+    if (plan->d_subf) (void)hipFree(plan->d_subf);
+    plan->d_subf = nullptr;
+    plan->sub_range = !(sub_start == 0 && sub_end == plan->limbs);
+    plan->sub_start = sub_start;
+    plan->sub_end = sub_end;
+    if (!sub_factors || w == 0) return NTT_OK;
+    std::vector<uint32_t> g(w);
+    for (uint32_t k = 0; k < w; ++k) {
+        // mont_mul(s, to_mont(g)) = s g mod q
+        const uint32_t q = plan->h_q[sub_start + k];
+        g[k] = to_mont_host(sub_factors[k] % q, q);
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&plan->d_subf), w * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        plan->d_subf = nullptr;
+        return record_hip_error(e, "hipMalloc(mod-down sub factors)");
+    }
+    e = hipMemcpy(plan->d_subf, g.data(), w * sizeof(uint32_t), hipMemcpyHostToDevice);
+    return e == hipSuccess ? NTT_OK : record_hip_error(e, "hipMemcpy(mod-down sub factors)");
+}
+
+int ntt_plan_forward_skip(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, uint32_t skip_start,
+                          uint32_t skip_end, size_t count, void* stream) {
+    if (!plan || skip_start > skip_end || skip_end > plan->limbs || skip_end - skip_start >= plan->limbs)
+        return NTT_ERR_INVALID_ARGUMENT;
+    LimbMap m;
+    m.skip_start = skip_start;
+    m.skip_len = skip_end - skip_start;
+    return plan_run(plan, dst_dev, count, false, stream, src_dev, nullptr, m);
+}
+
+int ntt_plan_inverse_from(ntt_plan* plan, uint32_t* dst_dev, const uint32_t* src_dev, uint32_t src_limbs,
+                          uint32_t src_first, size_t count, void* stream) {
+    if (!plan || !src_dev || src_limbs == 0 || (size_t)src_first + plan->limbs > src_limbs)
+        return NTT_ERR_INVALID_ARGUMENT;
+    LimbMap m;
+    m.src_period = src_limbs;
+    m.src_first = src_first;
+    return plan_run(plan, dst_dev, count, true, stream, src_dev, nullptr, m);
 }
 
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream) {

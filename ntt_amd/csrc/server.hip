@@ -273,7 +273,7 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
             for (size_t i : kv.second) {
                 const Req& q = *reqs[i];
                 FusedJob& j = s.h_desc[nd++];
-                j = FusedJob{};
+                j = fused_job_plain(q.r.limbs);
                 j.data = S->arena + q.off / sizeof(uint32_t);
                 j.src = j.data;
                 j.tw = inv ? q.plan->d_tw_inv : q.plan->d_tw_fwd;

@@ -108,6 +108,8 @@ __global__ __launch_bounds__(256) void static_setup_kernel(SetupArgs a) {
     }
     if (threadIdx.x == 0 && j.desc != NO_DESC) {
         FusedJob d{};
+        d.src_period = j.limbs;
+        d.sub_end = j.limbs;
         d.data = j.data;
         d.src = j.data;
         d.sub = nullptr;

@@ -156,16 +156,27 @@ def inverse(A: np.ndarray, logN: int, moduli, domain: int = MONTGOMERY) -> np.nd
     return out
 
 
-def mod_down(ntt_out: np.ndarray, sub: np.ndarray, factors, moduli) -> np.ndarray:
+def mod_down(ntt_out: np.ndarray, sub: np.ndarray, factors, moduli, sub_start: int = 0, sub_end=None,
+             sub_factors=None) -> np.ndarray:
     """The subtract-and-scale epilogue of Cheddar's NTTForModDown
-    (src/NTT_cheddar.cu:573-599: res = src2 - NTT, then res * inv_p_prod mod
-    the limb's prime): out[l] = (sub[l] - ntt_out[l]) * factors[l] mod q_l,
-    canonical, for (limbs, N) arrays.  The padding multiply of src2
-    (579-584) is not restated (the build has no counterpart)."""
+    (src/NTT_cheddar.cu:566-599): per limb l, s = src2 limb l - sub_start
+    when sub_start <= l < sub_end (src2_exists, 575-576), multiplied by
+    sub_factors[l - sub_start] when given (src2_padding, 579-584), else 0;
+    then res = s - NTT and res * inv_p_prod mod the limb's prime:
+    out[l] = (s - ntt_out[l]) * factors[l] mod q_l, canonical.  `sub` holds
+    sub_end - sub_start limbs (all limbs by default)."""
+    L = len(moduli)
+    sub_end = L if sub_end is None else sub_end
     qs = np.array([int(q) for q, _ in moduli], dtype=np.int64)[:, None]
     f = np.array([int(v) for v in factors], dtype=np.int64)[:, None] % qs
-    x = np.asarray(ntt_out, dtype=np.int64).reshape(len(moduli), -1)
-    y = np.asarray(sub, dtype=np.int64).reshape(len(moduli), -1)
+    x = np.asarray(ntt_out, dtype=np.int64).reshape(L, -1)
+    y = np.zeros_like(x)
+    if sub_end > sub_start:
+        sv = np.asarray(sub, dtype=np.int64).reshape(sub_end - sub_start, -1)
+        for l in range(sub_start, sub_end):
+            q = int(qs[l, 0])
+            g = 1 if sub_factors is None else int(sub_factors[l - sub_start]) % q
+            y[l] = sv[l - sub_start] % q * g % q       # < 2^31 * 2^31: exact in int64
     d = (y - x) % qs                          # in [0, q) < 2^31
     # d * f < 2^62: exact in int64
     return (d * f % qs).astype(np.uint32)

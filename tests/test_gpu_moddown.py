@@ -220,3 +220,97 @@ def test_mod_down_and_ot_graph_replays(env):
         plan.forward(fw, 1, s)          # eager work between replays
         s.synchronize()
     g.close()
+
+
+@pytest.mark.parametrize("logN,limbs", [(12, 5), (16, 4), (17, 3)])
+@pytest.mark.parametrize("domain", [0, 1])
+@pytest.mark.parametrize("padding", [False, True])
+def test_mod_down_sub_range(env, logN, limbs, domain, padding):
+    """Cheddar's src2 limb range and padding (NTT_cheddar.cu:566-590): sub
+    holds limbs [lo, hi) only (optionally multiplied by per-limb factors);
+    limbs outside the range store (0 - NTT) * f.  Two operands per call."""
+    ntt_amd, gu, orc = env
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    lo, hi = 1, limbs - 1
+    x = np.stack([gu.random_operand(logN, moduli, 80 + k) for k in range(2)])
+    sub = np.stack([gu.random_operand(logN, moduli[lo:hi], 90 + k) for k in range(2)])
+    f = [11 + 5 * l for l in range(limbs)]
+    g = [7919 + 13 * k for k in range(hi - lo)] if padding else None
+    for engine in _engines(logN):
+        _print("case mod-down range", logN, limbs, domain, padding, engine)
+        ctx, plan = _plan(ntt_amd, logN, moduli, domain, engine)
+        plan.set_mod_down_factors(f)
+        plan.set_mod_down_sub(lo, hi, g)
+        dst = ntt_amd.DeviceBuffer(x.nbytes)
+        plan.forward_mod_down(dst, gu.to_dev(x), gu.to_dev(sub), 2)
+        gu.sync()
+        got = gu.to_host(dst).reshape(x.shape)
+        for k in range(2):
+            exp = orc.mod_down(orc.forward(x[k], logN, moduli, domain), sub[k], f, moduli,
+                               sub_start=lo, sub_end=hi, sub_factors=g)
+            assert np.array_equal(got[k], exp), (engine, k)
+        # back to the whole range: the plain form again
+        plan.set_mod_down_sub(0, limbs, None)
+        full = np.stack([gu.random_operand(logN, moduli, 95 + k) for k in range(2)])
+        plan.forward_mod_down(dst, gu.to_dev(x), gu.to_dev(full), 2)
+        gu.sync()
+        got = gu.to_host(dst).reshape(x.shape)
+        assert np.array_equal(got[1], orc.mod_down(orc.forward(x[1], logN, moduli, domain), full[1], f, moduli))
+
+
+@pytest.mark.parametrize("logN,limbs", [(11, 6), (16, 5), (17, 4)])
+@pytest.mark.parametrize("skip", [(0, 2), (2, 3), (-2, 0)])
+def test_forward_skip_range(env, logN, limbs, skip):
+    """NTTForModUp's skipped limb range (NTT_cheddar.cu:815-887): limbs outside
+    [skip_start, skip_end) transformed in one launch, limbs inside untouched;
+    in place and out of place, two operands."""
+    ntt_amd, gu, orc = env
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
+    s0, s1 = skip if skip[0] >= 0 else (limbs + skip[0], limbs)      # (-2, 0): the last two limbs
+    x = np.stack([gu.random_operand(logN, moduli, 110 + k) for k in range(2)])
+    fill = np.full(x.shape, 0x5a5a5a5a, dtype=np.uint32)
+    for engine in _engines(logN):
+        for domain in (0, 1):
+            _print("case forward-skip", logN, limbs, skip, engine, domain)
+            ctx, plan = _plan(ntt_amd, logN, moduli, domain, engine)
+            exp = np.stack([orc.forward(x[k], logN, moduli, domain) for k in range(2)])
+            exp[:, s0:s1] = fill[:, s0:s1]
+            dst = gu.to_dev(fill)
+            plan.forward_skip(dst, gu.to_dev(x), s0, s1, 2)
+            gu.sync()
+            assert np.array_equal(gu.to_host(dst).reshape(x.shape), exp), (engine, "out of place")
+            A = gu.to_dev(x)
+            plan.forward_skip(A, None, s0, s1, 2)
+            gu.sync()
+            exp[:, s0:s1] = x[:, s0:s1]
+            assert np.array_equal(gu.to_host(A).reshape(x.shape), exp), (engine, "in place")
+
+
+@pytest.mark.parametrize("logN,limbs", [(13, 4), (16, 3), (17, 2)])
+def test_inverse_from_limb_subset(env, logN, limbs):
+    """INTTForModDown over a limb subset of its source (NTT_cheddar.cu:974-1049):
+    dst = INTT(limbs [first, first + L) of source operands of W limbs) * e_l,
+    two operands, both engines and domains."""
+    ntt_amd, gu, orc = env
+    W, first = limbs + 3, 2
+    wide = [tuple(m) for m in ntt_amd.generate_moduli(logN, W)]
+    moduli = wide[first:first + limbs]
+    src = np.stack([gu.random_operand(logN, wide, 130 + k) for k in range(2)])
+    e = [101 + 3 * l for l in range(limbs)]
+    for engine in _engines(logN):
+        for domain in (0, 1):
+            _print("case inverse-from", logN, limbs, engine, domain)
+            ctx, plan = _plan(ntt_amd, logN, moduli, domain, engine)
+            plan.set_constants(None, e)
+            dst = ntt_amd.DeviceBuffer(4 * 2 * limbs << logN)
+            S = gu.to_dev(src)
+            plan.inverse_from(dst, S, W, first, 2)
+            gu.sync()
+            got = gu.to_host(dst).reshape(2, limbs, 1 << logN)
+            for k in range(2):
+                inv = orc.inverse(src[k, first:first + limbs], logN, moduli, domain).astype(object)
+                exp = np.array([inv[l] * e[l] % moduli[l][0] for l in range(limbs)], dtype=np.uint32)
+                assert np.array_equal(got[k], exp), (engine, domain, k)
+            assert np.array_equal(gu.to_host(S).reshape(src.shape), src)      # source untouched
+            with pytest.raises(ntt_amd.NTTError):
+                plan.inverse_from(dst, S, W, W - limbs + 1, 2)                  # range past the source

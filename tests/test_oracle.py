@@ -273,6 +273,29 @@ def test_mod_down_exact_division(oracle_mod):
     assert int(d[l, i]) == (int(sub[l, i]) - int(ntt_x[l, i])) % q
 
 
+def test_mod_down_sub_range_and_padding(oracle_mod):
+    """src2 limb range and padding (NTT_cheddar.cu:566-590), pinned by the same
+    exact division: with padding g_l and sub limb S_l = NTT(x + P y)_l * g_l^-1
+    for l in the range, the result is NTT(y)_l; outside the range it is
+    (0 - NTT(x)_l) * f_l = NTT(-x P^-1)_l by linearity of the transform."""
+    ntt_x, ntt_xpy, f, moduli, ntt_y = _mod_down_case(oracle_mod)
+    L = len(moduli)
+    qs = [q for q, _ in moduli]
+    lo, hi = 1, 4
+    g = [7919 + 13 * k for k in range(hi - lo)]
+    sub = np.array([ntt_xpy[l].astype(object) * pow(g[l - lo], -1, qs[l]) % qs[l] for l in range(lo, hi)],
+                   dtype=np.uint32)
+    got = oracle_mod.mod_down(ntt_x, sub, f, moduli, sub_start=lo, sub_end=hi, sub_factors=g)
+    x = oracle_mod.inverse(ntt_x, 10, moduli, oracle_mod.MONTGOMERY)     # back to x
+    mx = np.array([(-(x[l].astype(object)) * f[l]) % qs[l] for l in range(L)], dtype=np.uint32)
+    ntt_mx = oracle_mod.forward(mx, 10, moduli, oracle_mod.MONTGOMERY)
+    for l in range(L):
+        assert np.array_equal(got[l], ntt_y[l] if lo <= l < hi else ntt_mx[l]), l
+    # a padding applied to the wrong sub limb (off by one) fails
+    bad = oracle_mod.mod_down(ntt_x, sub, f, moduli, sub_start=lo, sub_end=hi, sub_factors=g[1:] + g[:1])
+    assert not np.array_equal(bad[lo], ntt_y[lo])
+
+
 def test_planted_mistake_mod_down_operand_order_is_caught(oracle_mod):
     """NTT - sub instead of sub - NTT (a sign slip) gives NTT(-y), not NTT(y)."""
     ntt_x, sub, f, moduli, ntt_y = _mod_down_case(oracle_mod)
