@@ -25,6 +25,7 @@
 #include "host_math.hpp"
 #include "launch.hpp"
 #include "phase_kernels.hpp"
+#include "trace.hpp"
 
 namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
@@ -340,6 +341,7 @@ int ntt_batch_destroy(ntt_batch* batch) {
 
 int ntt_batch_run(ntt_batch* batch, void* stream) {
     if (!batch) return NTT_ERR_INVALID_ARGUMENT;
+    TraceRange range("ntt_batch_run");
     return enqueue(batch, (hipStream_t)stream);
 }
 
@@ -366,6 +368,7 @@ int ntt_batch_capture(ntt_batch* batch, void* stream) {
 
 int ntt_batch_replay(ntt_batch* batch, void* stream) {
     if (!batch) return NTT_ERR_INVALID_ARGUMENT;
+    TraceRange range("ntt_batch_replay");
     int rc = NTT_OK;
     if (!batch->exec && (rc = ntt_batch_capture(batch, stream))) return rc;
     return ck(hipGraphLaunch(batch->exec, (hipStream_t)stream), "hipGraphLaunch");

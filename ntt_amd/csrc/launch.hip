@@ -14,6 +14,7 @@
 #include "host_math.hpp"
 #include "launch.hpp"
 #include "phase_kernels.hpp"
+#include "trace.hpp"
 
 namespace ntt {
 
@@ -215,6 +216,7 @@ const char* ntt_last_hip_error(void) { return g_last_error; }
 int ntt_launch_ntt_montgomery(uint32_t* A_dev, const uint32_t* twiddle_dev, const uint32_t* moduli_dev,
                               const uint32_t* q_inv_dev, const uint32_t* r2_dev, size_t limb_count, int logN,
                               const ntt_transform_config* cfg, void* stream) {
+    TraceRange range("custom_ntt");
     TransformLaunch L{};
     int rc = resolve_cfg(logN, cfg, &L.cfg);
     if (rc) return rc;
@@ -228,6 +230,7 @@ int ntt_launch_ntt_montgomery(uint32_t* A_dev, const uint32_t* twiddle_dev, cons
 int ntt_launch_intt_montgomery(uint32_t* A_dev, const uint32_t* inv_twiddle_dev, const uint32_t* moduli_dev,
                                const uint32_t* q_inv_dev, const uint32_t* degree_inv_dev, size_t limb_count,
                                int logN, const ntt_transform_config* cfg, void* stream) {
+    TraceRange range("custom_intt");
     TransformLaunch L{};
     int rc = resolve_cfg(logN, cfg, &L.cfg);
     if (rc) return rc;
@@ -241,6 +244,7 @@ int ntt_launch_intt_montgomery(uint32_t* A_dev, const uint32_t* inv_twiddle_dev,
 
 int ntt_launch_ntt_plain(uint32_t* A_dev, const uint32_t* twiddle_dev, const uint32_t* moduli_dev,
                          size_t limb_count, int logN, const ntt_transform_config* cfg, void* stream) {
+    TraceRange range("custom_ntt");
     TransformLaunch L{};
     int rc = resolve_cfg(logN, cfg, &L.cfg);
     if (rc) return rc;
@@ -253,6 +257,7 @@ int ntt_launch_ntt_plain(uint32_t* A_dev, const uint32_t* twiddle_dev, const uin
 int ntt_launch_intt_plain(uint32_t* A_dev, const uint32_t* inv_twiddle_dev, const uint32_t* moduli_dev,
                           const uint32_t* degree_inv_dev, size_t limb_count, int logN,
                           const ntt_transform_config* cfg, void* stream) {
+    TraceRange range("custom_intt");
     TransformLaunch L{};
     int rc = resolve_cfg(logN, cfg, &L.cfg);
     if (rc) return rc;

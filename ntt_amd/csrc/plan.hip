@@ -20,6 +20,7 @@
 #include "host_math.hpp"
 #include "launch.hpp"
 #include "phase_kernels.hpp"
+#include "trace.hpp"
 
 namespace ntt {
 
@@ -299,6 +300,7 @@ static bool overlap(const void* a, size_t abytes, const void* b, size_t bbytes) 
 static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse, void* stream,
                     const uint32_t* src = nullptr, const uint32_t* sub = nullptr, const LimbMap& map = LimbMap()) {
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
+    TraceRange range(sub ? "ntt_forward_mod_down" : inverse ? "ntt_inverse" : "ntt_forward");
     const size_t N = (size_t)1 << plan->logN;
     const size_t bytes = plan->limbs * N * count * sizeof(uint32_t);
     if (src == A_dev && !map.src_period && !map.src_first) src = nullptr;

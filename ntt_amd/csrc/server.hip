@@ -36,6 +36,7 @@
 #include "host_math.hpp"
 #include "launch.hpp"
 #include "phase_kernels.hpp"
+#include "trace.hpp"
 
 namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
@@ -222,6 +223,7 @@ int plan_for(ntt_server* S, const ntt_request& r, ntt_plan** out) {
 // by this launch until the slot retires).
 int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req>>& reqs, uint64_t* fused,
                    uint64_t* two_pass) {
+    TraceRange range("ntt_server_launch");
     int rc = NTT_OK;
     // 1. operands into the arena: copies, and every seeded fill in one launch
     uint32_t nfill = 0;

@@ -28,6 +28,7 @@
 #include "host_math.hpp"
 #include "launch.hpp"
 #include "phase_kernels.hpp"
+#include "trace.hpp"
 
 namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
@@ -164,6 +165,7 @@ int ntt_batch_run_static(const ntt_table_job* jobs, size_t njobs, void* stream) 
         total_limbs += j.limbs;
         if (total_limbs > MAX_LIMBS) return NTT_ERR_SCRATCH_LIMIT;
     }
+    TraceRange range("ntt_batch_run_static");
     unsigned char* base = nullptr;
     int rc = scratch_for_current_device(&base);
     if (rc) return rc;

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <vector>
 
+#include "trace.hpp"
 #include "../../include/ntt_amd.h"
 #include "arith64.hpp"
 #include "context.hpp"
@@ -314,6 +315,7 @@ int ntt_plan64_destroy(ntt_plan64* plan) {
 }
 
 int ntt_plan64_forward(ntt_plan64* plan, uint64_t* A_dev, size_t count, void* stream) {
+    TraceRange range("ntt64_forward");
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
     int rc = launch_pass64(plan, A_dev, count, false, true, (hipStream_t)stream);
     if (rc == NTT_OK) rc = launch_pass64(plan, A_dev, count, false, false, (hipStream_t)stream);
@@ -321,6 +323,7 @@ int ntt_plan64_forward(ntt_plan64* plan, uint64_t* A_dev, size_t count, void* st
 }
 
 int ntt_plan64_inverse(ntt_plan64* plan, uint64_t* A_dev, size_t count, void* stream) {
+    TraceRange range("ntt64_inverse");
     if (!plan || !A_dev || count == 0) return NTT_ERR_INVALID_ARGUMENT;
     int rc = launch_pass64(plan, A_dev, count, true, false, (hipStream_t)stream);
     if (rc == NTT_OK) rc = launch_pass64(plan, A_dev, count, true, true, (hipStream_t)stream);
