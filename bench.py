@@ -638,12 +638,13 @@ def main():
         log("ROUNDTRIP MISMATCH after the timed region")
 
     # ---- kernel durations of the engine in use (HIP events on the launch stream) ----
-    ebf, lzf, lwf = plan.fused_variant_full(B, ntt_amd.FORWARD)
-    ebi, lzi, lwi = plan.fused_variant_full(B, ntt_amd.INVERSE)
+    ebf, lzf, lwf, otf = plan.fused_variant_full(B, ntt_amd.FORWARD)
+    ebi, lzi, lwi, oti = plan.fused_variant_full(B, ntt_amd.INVERSE)
     lazy = all(q < (1 << 30) for q, _ in moduli)   # two-pass kernels then use lazy reduction
     if engine_used == "fused":
-        kern = {f"fused_kernel<{logN}, false, {'true' if lzf else 'false'}, {ebf}, false, false, {lwf}>": lambda: plan.forward(pool, B, stream),
-                f"fused_kernel<{logN}, true, {'true' if lzi else 'false'}, {ebi}, false, false, {lwi}>": lambda: plan.inverse(pool, B, stream)}
+        tf = lambda b: "true" if b else "false"
+        kern = {f"fused_kernel<{logN}, false, {tf(lzf)}, {ebf}, false, {tf(otf)}, {lwf}>": lambda: plan.forward(pool, B, stream),
+                f"fused_kernel<{logN}, true, {tf(lzi)}, {ebi}, false, {tf(oti)}, {lwi}>": lambda: plan.inverse(pool, B, stream)}
     else:
         mode = 2 if lazy else 0
         c = cfg.astuple()
@@ -785,7 +786,7 @@ def main():
                     ev_b.record(stream)
                     ev_b.synchronize()
                     ot[f"{tag}_{name}_ms"] = round(ev_a.elapsed_ms(ev_b) / 5, 4)
-            plan.set_twiddle_mode(ntt_amd.TWIDDLE_TABLE)
+            plan.set_twiddle_mode(ntt_amd.TWIDDLE_AUTO)
             extras["twiddle_ot"] = ot
         extras["word64"] = bench_word64(ntt_amd, dev, stream, logN, limbs)
         log("extras: " + json.dumps(extras))
@@ -838,7 +839,9 @@ def main():
                         f"({4 * words_per_op * B / 2**30:.2f} GiB, HBM-resident), batched launch",
             "logN": logN, "limbs": limbs, "batch": B,
             "engine": engine_used,
-            "fused_variant": {"forward": [ebf, lzf, lwf], "inverse": [ebi, lzi, lwi]} if engine_used == "fused" else None,
+            "fused_variant": {"forward": {"eb": ebf, "lazy": lzf, "log2_words": lwf, "on_the_fly_twiddles": otf},
+                              "inverse": {"eb": ebi, "lazy": lzi, "log2_words": lwi, "on_the_fly_twiddles": oti}}
+                             if engine_used == "fused" else None,
             "phase_config": cfg.astuple(),
             "chunk_operands": tuning["batch"]["chunk"] if tuning else 0,
             "parallelism": "single GPU",

@@ -60,8 +60,9 @@ typedef enum { NTT_ENGINE_AUTO = 0, NTT_ENGINE_TWO_PASS = 1, NTT_ENGINE_FUSED = 
  * their own (the upper levels of the table).  TABLE: from the table.  OT: on
  * the fly, tw[i] = mont_mul(tw[i mod 64], msb[i / 64]), from the table's
  * first 64 words and an N/64-word appendix msb[h] = tw[64 h] that the
- * context stores after every table of 2^10 words or more. */
-typedef enum { NTT_TWIDDLE_TABLE = 0, NTT_TWIDDLE_OT = 1 } ntt_twiddle_mode;
+ * context stores after every table of 2^10 words or more.  AUTO (the
+ * default): the tuned choice for the launch size, else OT. */
+typedef enum { NTT_TWIDDLE_AUTO = -1, NTT_TWIDDLE_TABLE = 0, NTT_TWIDDLE_OT = 1 } ntt_twiddle_mode;
 
 /* ModulusConfig, src/ntt_config.h:6-9 */
 typedef struct {
@@ -223,18 +224,19 @@ int ntt_plan_set_fused_words(ntt_plan* plan, int log2_words);
  * twiddle, NTT_cheddar.cu:1283-1289; MultiRadixNTT_OT / MultiRadixINTT_OT in
  * the last stage group, 79-93 and 421-435).  A fused launch in this mode
  * reads 2^10 + N/64 table words per limb in place of N, at one Montgomery
- * multiply per twiddle of that round; shapes are fixed (forward 7 lazy when
- * every q < 2^30, else 6 canonical; inverse 6 canonical).  Two-pass launches
+ * multiply per twiddle of that round; the shape is 1024 threads x 64 words,
+ * with the plan's arithmetic (lazy when every q < 2^30).  Two-pass launches
  * and the mod-down form read the table as before. */
 int ntt_plan_set_twiddle_mode(ntt_plan* plan, int mode);
 /* Arithmetic of the fused kernels, both directions: 1 = lazy reduction (used
  * only when every q < 2^30), 0 = canonical, -1 = the tuned choice (default
  * lazy).  Outputs are canonical either way. */
 int ntt_plan_set_fused_arith(ntt_plan* plan, int lazy);
-/* The fused kernel variant (shape, lazy, words per workgroup as log2) a run
- * of `count` operands uses in `direction`; any output may be NULL. */
+/* The fused kernel variant (shape, lazy, words per workgroup as log2,
+ * on-the-fly twiddles) a run of `count` operands uses in `direction`; any
+ * output may be NULL. */
 int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction, int* eb, int* lazy,
-                               int* log2_words);
+                               int* log2_words, int* on_the_fly);
 /* Engine a launch of `count` operands would use (AUTO resolved). */
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count);
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream);
@@ -351,6 +353,8 @@ typedef struct {
     int fused_lazy_inv;         /* fused inverse arithmetic: 1 lazy, 0 canonical */
     int fused_words_log2;       /* fused forward words per workgroup (log2): 16 or 15 */
     int fused_words_log2_inv;   /* fused inverse words per workgroup (log2) */
+    int fused_ot;               /* fused forward twiddles: 1 on the fly, 0 table */
+    int fused_ot_inv;           /* fused inverse twiddles: 1 on the fly, 0 table */
 } ntt_tune_result;
 
 int ntt_autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs,

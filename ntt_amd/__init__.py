@@ -45,7 +45,7 @@ STATUS = {
 PLAIN, MONTGOMERY = 0, 1
 FORWARD, INVERSE = 0, 1
 ENGINE_AUTO, ENGINE_TWO_PASS, ENGINE_FUSED = 0, 1, 2
-TWIDDLE_TABLE, TWIDDLE_OT = 0, 1
+TWIDDLE_AUTO, TWIDDLE_TABLE, TWIDDLE_OT = -1, 0, 1
 
 
 class NTTError(RuntimeError):
@@ -130,7 +130,7 @@ class TuneResult(ctypes.Structure):
                 ("fused_fwd_ms", ctypes.c_double), ("fused_inv_ms", ctypes.c_double), ("fused_stagger", ctypes.c_int),
                 ("fused_eb", ctypes.c_int), ("fused_eb_inv", ctypes.c_int), ("fused_lazy", ctypes.c_int),
                 ("fused_lazy_inv", ctypes.c_int), ("fused_words_log2", ctypes.c_int),
-                ("fused_words_log2_inv", ctypes.c_int)]
+                ("fused_words_log2_inv", ctypes.c_int), ("fused_ot", ctypes.c_int), ("fused_ot_inv", ctypes.c_int)]
 
     def as_dict(self):
         return {"cfg": self.cfg.astuple(), "chunk": int(self.chunk), "fwd_ms": self.fwd_ms, "inv_ms": self.inv_ms,
@@ -139,7 +139,8 @@ class TuneResult(ctypes.Structure):
                 "fused_fwd_ms": self.fused_fwd_ms, "fused_inv_ms": self.fused_inv_ms,
                 "fused_stagger": self.fused_stagger, "fused_eb": self.fused_eb, "fused_eb_inv": self.fused_eb_inv,
                 "fused_lazy": self.fused_lazy, "fused_lazy_inv": self.fused_lazy_inv,
-                "fused_words_log2": self.fused_words_log2, "fused_words_log2_inv": self.fused_words_log2_inv}
+                "fused_words_log2": self.fused_words_log2, "fused_words_log2_inv": self.fused_words_log2_inv,
+                "fused_ot": self.fused_ot, "fused_ot_inv": self.fused_ot_inv}
 
 
 class BatchJob(ctypes.Structure):
@@ -247,7 +248,8 @@ _PROTOS = {
     "ntt_plan_set_fused_arith": (_I, [_P, _I]),
     "ntt_plan_set_twiddle_mode": (_I, [_P, _I]),
     "ntt_plan_set_constants": (_I, [_P, _PU32, _PU32]),
-    "ntt_plan_get_fused_variant": (_I, [_P, _SZ, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I)]),
+    "ntt_plan_get_fused_variant": (_I, [_P, _SZ, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                                        ctypes.POINTER(_I)]),
     "ntt_plan_get_engine": (_I, [_P, _SZ]),
     "ntt_autotune": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _SZ, _I, _I, ctypes.POINTER(TuneResult)]),
     "ntt_context_set_tuning": (_I, [_P, _I, _SZ, _PCFG, _SZ, _I]),
@@ -779,11 +781,11 @@ class Plan:
         return self.fused_variant_full(count, direction)[:2]
 
     def fused_variant_full(self, count: int = 1, direction: int = FORWARD):
-        """(shape, lazy, log2 words per workgroup) of the fused kernel."""
-        eb, lz, lw = _I(0), _I(0), _I(0)
+        """(shape, lazy, log2 words per workgroup, on-the-fly twiddles) of the fused kernel."""
+        eb, lz, lw, ot = _I(0), _I(0), _I(0), _I(0)
         _check(_lib.ntt_plan_get_fused_variant(self.handle, count, direction, ctypes.byref(eb), ctypes.byref(lz),
-                                               ctypes.byref(lw)), "ntt_plan_get_fused_variant")
-        return eb.value, bool(lz.value), lw.value
+                                               ctypes.byref(lw), ctypes.byref(ot)), "ntt_plan_get_fused_variant")
+        return eb.value, bool(lz.value), lw.value, bool(ot.value)
 
     def set_constants(self, prologue=None, epilogue=None):
         """Fused per-limb constant multiplies: forward computes NTT(prologue * x),
@@ -798,11 +800,11 @@ class Plan:
                "ntt_plan_set_constants")
 
     def set_twiddle_mode(self, mode: int):
-        """TWIDDLE_TABLE (default) or TWIDDLE_OT: on-the-fly twiddles in the fused engine."""
+        """TWIDDLE_AUTO (default: the tuned choice, else on the fly), TWIDDLE_TABLE or TWIDDLE_OT."""
         _check(_lib.ntt_plan_set_twiddle_mode(self.handle, mode), "ntt_plan_set_twiddle_mode")
 
     def set_fused_arith(self, lazy: int):
-        """Fused forward arithmetic: 1 lazy (when every q < 2^30), 0 canonical, -1 tuned."""
+        """Fused arithmetic, both directions: 1 lazy (when every q < 2^30), 0 canonical, -1 tuned."""
         _check(_lib.ntt_plan_set_fused_arith(self.handle, lazy), "ntt_plan_set_fused_arith")
 
     def set_fused_words(self, log2_words: int):

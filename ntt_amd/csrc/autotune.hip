@@ -207,19 +207,22 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
             }
         }
         // the fused limb kernel where it applies: each direction on its own,
-        // over both window widths, both arithmetics when every q < 2^30, and
-        // for logN <= 15 both workgroup sizes (2^16 words, one per CU; 2^15
-        // words, two per CU)
+        // over both window widths, both arithmetics when every q < 2^30, for
+        // logN <= 15 both workgroup sizes (2^16 words, one per CU; 2^15
+        // words, two per CU), and table or on-the-fly twiddles (the latter
+        // at width 6, 2^16 words)
         if (rc == NTT_OK && fused_supported(logN)) {
             double bf = 0, bi = 0;
             for (int logw = 16; rc == NTT_OK && logw >= (logN <= 15 ? 15 : 16); --logw)
                 for (int eb = 7; rc == NTT_OK && eb >= 6; --eb)
+                    for (int ot = 0; rc == NTT_OK && ot <= (eb == 6 && logw == 16 ? 1 : 0); ++ot)
                     for (int lz = P.lazy ? 1 : 0; rc == NTT_OK && lz >= 0; --lz)
                         for (int inv = 0; rc == NTT_OK && inv < 2; ++inv) {
                             FusedVariant v;
                             v.eb = eb;
                             v.lazy = lz != 0;
                             v.logw = logw;
+                            v.ot = ot != 0;
                             double t = 0;
                             rc = time_steady(s, reps + 2,
                                              [&] { return plan_launch_fused(&P, buf, count, inv != 0, s, v, 0); }, &t);
@@ -278,6 +281,8 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
     e.fused_lazy_inv = vi.lazy;
     e.fused_logw_fwd = vf.logw;
     e.fused_logw_inv = vi.logw;
+    e.fused_ot_fwd = vf.ot;
+    e.fused_ot_inv = vi.ot;
     e.stagger = stagger;
     e.chunk = best_chunk;
     e.fwd_ms = best_fwd;
@@ -304,6 +309,8 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
         out->fused_lazy_inv = vi.lazy ? 1 : 0;
         out->fused_words_log2 = vf.logw;
         out->fused_words_log2_inv = vi.logw;
+        out->fused_ot = vf.ot ? 1 : 0;
+        out->fused_ot_inv = vi.ot ? 1 : 0;
     }
     return NTT_OK;
 }
@@ -350,16 +357,17 @@ int ntt_context_save_tuning(const ntt_context* ctx, const char* path) {
     FILE* f = std::fopen(path, "w");
     if (!f) return NTT_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> g(ctx->tune_mu);
-    std::fprintf(f, "# ntt_amd tuning v3: logN limbs_log2 chunk r s w x4 fwd_ms inv_ms engine fused_eb_fwd "
-                    "fused_eb_inv fused_lazy stagger fused_lazy_inv fused_logw_fwd fused_logw_inv\n");
+    std::fprintf(f, "# ntt_amd tuning v4: logN limbs_log2 chunk r s w x4 fwd_ms inv_ms engine fused_eb_fwd "
+                    "fused_eb_inv fused_lazy stagger fused_lazy_inv fused_logw_fwd fused_logw_inv fused_ot_fwd "
+                    "fused_ot_inv\n");
     for (auto& kv : ctx->tuned) {
         const TunedEntry& e = kv.second;
         const ntt_phase_config* p = &e.cfg.ntt_phase1;
         std::fprintf(f, "%d %d %zu", kv.first.first, kv.first.second, e.chunk);
         for (int i = 0; i < 4; ++i) std::fprintf(f, " %d %d %d", p[i].radix_stages, p[i].stage_merging, p[i].warp_batching);
-        std::fprintf(f, " %.6f %.6f %d %d %d %d %d %d %d %d\n", e.fwd_ms, e.inv_ms, e.engine, e.fused_eb_fwd,
+        std::fprintf(f, " %.6f %.6f %d %d %d %d %d %d %d %d %d %d\n", e.fwd_ms, e.inv_ms, e.engine, e.fused_eb_fwd,
                      e.fused_eb_inv, e.fused_lazy ? 1 : 0, e.stagger, e.fused_lazy_inv ? 1 : 0, e.fused_logw_fwd,
-                     e.fused_logw_inv);
+                     e.fused_logw_inv, e.fused_ot_fwd ? 1 : 0, e.fused_ot_inv ? 1 : 0);
     }
     std::fclose(f);
     return NTT_OK;
@@ -377,10 +385,11 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         size_t chunk;
         double fm = 0, im = 0;
         int eng = NTT_ENGINE_TWO_PASS, febf = FUSED_DEFAULT_EB_FWD, febi = FUSED_DEFAULT_EB_INV, lz = 1, stag = 0;
-        int lzi = 1, lwf = 16, lwi = 16;
-        int n = std::sscanf(line, "%d %d %zu %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d %d %d %d %d %d %d %d",
+        int lzi = 1, lwf = 16, lwi = 16, otf = 0, oti = 0;
+        int n = std::sscanf(line,
+                            "%d %d %zu %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d %d %d %d %d %d %d %d %d %d",
                             &logN, &lb, &chunk, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9],
-                            &v[10], &v[11], &fm, &im, &eng, &febf, &febi, &lz, &stag, &lzi, &lwf, &lwi);
+                            &v[10], &v[11], &fm, &im, &eng, &febf, &febi, &lz, &stag, &lzi, &lwf, &lwi, &otf, &oti);
         if (n < 15) { rc = NTT_ERR_INVALID_ARGUMENT; break; }
         TunedEntry e;
         ntt_phase_config* p = &e.cfg.ntt_phase1;
@@ -394,6 +403,8 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         e.fused_lazy_inv = lzi != 0;
         e.fused_logw_fwd = lwf == 15 && logN <= 15 ? 15 : 16;
         e.fused_logw_inv = lwi == 15 && logN <= 15 ? 15 : 16;
+        e.fused_ot_fwd = otf != 0;
+        e.fused_ot_inv = oti != 0;
         e.stagger = stag < 0 ? 0 : (stag > 64 ? 64 : stag);
         e.fwd_ms = fm;
         e.inv_ms = im;

@@ -29,7 +29,7 @@
 
 namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
-                 int logw = 16);
+                 int logw = 16, bool ot = false);
 int fused_prepare();
 uint32_t fused_items(int logN, size_t limbs, int logw = 16);
 }  // namespace ntt
@@ -137,7 +137,7 @@ int enqueue(ntt_batch* b, hipStream_t stream) {
                 p.total_items = g.items;
                 p.counter = nullptr;          // static round-robin: nothing to reset between replays
                 p.stagger = 0;
-                rc = launch_fused(g.n, g.inverse, g.v.eb, g.v.lazy, p, s, 0, g.v.logw);
+                rc = launch_fused(g.n, g.inverse, g.v.eb, g.v.lazy, p, s, 0, g.v.logw, g.v.ot);
             } else {
                 const TwoPassGroup& g = b->tp_groups[(size_t)u.tp_group];
                 TransformLaunch L{};

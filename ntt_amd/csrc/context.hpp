@@ -49,6 +49,7 @@ struct TunedEntry {
     bool fused_lazy = true;             // fused forward arithmetic: lazy (when every q < 2^30)
     bool fused_lazy_inv = true;         // fused inverse arithmetic: lazy (when every q < 2^30)
     int fused_logw_fwd = 16, fused_logw_inv = 16;   // fused words per workgroup (log2) per direction
+    bool fused_ot_fwd = true, fused_ot_inv = true;  // fused on-the-fly twiddles per direction
     int stagger = 0;                    // fused engine: start stagger (FusedParams::stagger)
     size_t chunk = 0;          // operands per chunk, 0 = whole launch
     double fwd_ms = 0, inv_ms = 0;
@@ -93,7 +94,7 @@ struct ntt_plan {
     std::vector<uint32_t> h_scale_fwd, h_scale_inv;    // host copies of the scale words
     bool prologue = false;                         // forward input multiplied by per-limb constants
     uint32_t* d_post = nullptr;                    // mod-down multipliers to_mont(f_l), own allocation
-    bool twiddle_ot = false;                       // fused engine: on-the-fly twiddles (NTT_TWIDDLE_OT)
+    int twiddle_mode = NTT_TWIDDLE_AUTO;           // fused engine twiddles: ntt_twiddle_mode
     // mod-down sub operand: limbs [sub_start, sub_end) only (when sub_range),
     // each multiplied by d_subf[l - sub_start] = to_mont(factor) when non-null
     bool sub_range = false;

@@ -40,7 +40,7 @@ FusedFn fused_fn_ot(int n, bool inv, bool lazy);     // fused_ot.hip
 FusedFn fused_fn_w15(int n, bool inv, int eb, bool lazy);   // fused_w15.hip
 
 // Window width of the on-the-fly twiddle kernels.
-static int ot_eb(bool inv, bool lazy) { return !inv && lazy ? 7 : 6; }
+constexpr int OT_EB = 6;
 
 // Window width of the mod-down kernels: 7 with lazy arithmetic, 6 canonical.
 static int md_eb(bool lazy) { return lazy ? 7 : 6; }
@@ -66,8 +66,8 @@ int fused_prepare() {
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
             }
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
-            for (int d = 0; d < 3 && attr_err == hipSuccess; ++d) {
-                FusedFn f = fused_fn_ot(m, d == 2, d == 1);
+            for (int d = 0; d < 4 && attr_err == hipSuccess; ++d) {
+                FusedFn f = fused_fn_ot(m, d & 1, d >= 2);
                 if (f)
                     attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
@@ -111,9 +111,15 @@ static int launch_fused_fn(FusedFn fn, int eb, const FusedParams& p, hipStream_t
 // Launches the persistent kernel of size 2^n / direction `inv`; every job in
 // p has that size and direction.
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream,
-                 unsigned grid_limit, int logw) {
+                 unsigned grid_limit, int logw, bool ot) {
     if (p.total_items == 0) return NTT_OK;
     if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
+    if (ot) {
+        // on-the-fly twiddles: tables with the msb appendix, 1024 x 64 words per CU
+        FusedFn f = fused_fn_ot(n, inv, lazy);
+        if (!f) return NTT_ERR_CONFIG;
+        return launch_fused_fn(f, OT_EB, p, stream, grid_limit);
+    }
     FusedFn fn = fused_fn(n, inv, eb, lazy, logw);
     if (!fn) return NTT_ERR_CONFIG;
     return launch_fused_fn(fn, eb, p, stream, grid_limit, logw);
@@ -154,17 +160,17 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.job0.inverse = inverse ? 1 : 0;
     p.jobs = nullptr;
     p.njobs = 0;
-    const bool w15 = v.logw == 15 && !P->twiddle_ot && !sub && P->logN <= 15;
+    const bool w15 = v.logw == 15 && !v.ot && !sub && P->logN <= 15;
     p.total_items = fused_items(P->logN, limbs, w15 ? 15 : 16);
     p.counter = nullptr;
     p.stagger = stagger < 0 ? 0u : (uint32_t)stagger;
-    if (P->twiddle_ot && !sub) {
-        // on-the-fly twiddles: fixed shapes, lazy forward when every q < 2^30
-        const bool lz = P->lazy && !inverse;
-        FusedFn fn = fused_fn_ot(P->logN, inverse, lz);
+    if (v.ot && !sub) {
+        // on-the-fly twiddles: 1024 x 64 words, lazy when every q < 2^30 (and
+        // the variant asks for it)
+        FusedFn fn = fused_fn_ot(P->logN, inverse, v.lazy && P->lazy);
         if (!fn) return NTT_ERR_CONFIG;
         if (p.total_items == 0) return NTT_OK;
-        return launch_fused_fn(fn, ot_eb(inverse, lz), p, stream, 0);
+        return launch_fused_fn(fn, OT_EB, p, stream, 0);
     }
     if (sub) {
         const bool lz = P->lazy;   // the mod-down shapes: lazy when every q < 2^30
@@ -174,7 +180,7 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
         return launch_fused_fn(fn, md_eb(lz), p, stream, 0);
     }
     // lazy reduction needs every q < 2^30
-    return launch_fused(P->logN, inverse, v.eb, v.lazy && P->lazy, p, stream, 0, w15 ? 15 : 16);
+    return launch_fused(P->logN, inverse, v.eb, v.lazy && P->lazy, p, stream, 0, w15 ? 15 : 16, false);
 }
 
 }  // namespace ntt

@@ -24,6 +24,7 @@ struct FusedVariant {
     int eb = 6;          // window width: 6 or 7
     bool lazy = false;   // lazy reduction (every q < 2^30)
     int logw = 16;       // words per workgroup (log2): 16, or 15 (logN <= 15)
+    bool ot = false;     // on-the-fly twiddles in the low-bit round (eb 6, logw 16; tables with the msb appendix)
 };
 
 // On-the-fly twiddles: a cached table of N = 2^n words (n >= 10) is followed

@@ -179,6 +179,11 @@ static FusedVariant variant_of(const TunedEntry* t, int logN, bool inverse, bool
     v.lazy = (t ? (inverse ? t->fused_lazy_inv : t->fused_lazy) : true) && lazy_capable;
     v.logw = t ? (inverse ? t->fused_logw_inv : t->fused_logw_fwd) : 16;
     if (logN > 15) v.logw = 16;
+    v.ot = t ? (inverse ? t->fused_ot_inv : t->fused_ot_fwd) : true;
+    if (v.ot) {   // the on-the-fly twiddle kernels: 1024 x 64 words per CU
+        v.eb = 6;
+        v.logw = 16;
+    }
     return v;
 }
 
@@ -193,13 +198,17 @@ void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, FusedVari
     TunedEntry t;
     const bool tuned = !P->cfg_fixed && P->ctx->tuned_lookup(P->logN, P->limbs * count, &t);
     *v = variant_of(tuned ? &t : nullptr, P->logN, inverse, P->lazy);
+    if (P->twiddle_mode != NTT_TWIDDLE_AUTO) v->ot = P->twiddle_mode == NTT_TWIDDLE_OT;
+    // an explicit shape or word count other than the on-the-fly kernels' one
+    // selects table twiddles unless the caller asked for on-the-fly ones
+    if ((P->fused_eb && P->fused_eb != 6) || (P->fused_logw && P->fused_logw != 16 && P->logN <= 15))
+        if (P->twiddle_mode != NTT_TWIDDLE_OT) v->ot = false;
     if (P->fused_eb) v->eb = P->fused_eb;
     if (P->fused_arith >= 0) v->lazy = P->fused_arith == 1 && P->lazy;
     if (P->fused_logw) v->logw = P->logN <= 15 ? P->fused_logw : 16;
     *stagger = tuned ? t.stagger : 0;
-    if (P->twiddle_ot) {   // fixed shapes of the on-the-fly twiddle kernels
-        v->lazy = P->lazy && !inverse;
-        v->eb = v->lazy ? 7 : 6;
+    if (v->ot) {   // the on-the-fly twiddle kernels: 1024 x 64 words per CU
+        v->eb = 6;
         v->logw = 16;
     }
 }
@@ -398,8 +407,9 @@ int ntt_plan_set_constants(ntt_plan* plan, const uint32_t* prologue, const uint3
 }
 
 int ntt_plan_set_twiddle_mode(ntt_plan* plan, int mode) {
-    if (!plan || (mode != NTT_TWIDDLE_TABLE && mode != NTT_TWIDDLE_OT)) return NTT_ERR_INVALID_ARGUMENT;
-    plan->twiddle_ot = mode == NTT_TWIDDLE_OT;
+    if (!plan || (mode != NTT_TWIDDLE_TABLE && mode != NTT_TWIDDLE_OT && mode != NTT_TWIDDLE_AUTO))
+        return NTT_ERR_INVALID_ARGUMENT;
+    plan->twiddle_mode = mode;
     return NTT_OK;
 }
 
@@ -410,7 +420,7 @@ int ntt_plan_set_fused_arith(ntt_plan* plan, int lazy) {
 }
 
 int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction, int* eb, int* lazy,
-                               int* log2_words) {
+                               int* log2_words, int* on_the_fly) {
     if (!plan || (direction != NTT_FORWARD && direction != NTT_INVERSE)) return NTT_ERR_INVALID_ARGUMENT;
     int st = 0;
     FusedVariant v;
@@ -418,6 +428,7 @@ int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction
     if (eb) *eb = v.eb;
     if (lazy) *lazy = v.lazy ? 1 : 0;
     if (log2_words) *log2_words = v.logw;
+    if (on_the_fly) *on_the_fly = v.ot ? 1 : 0;
     return NTT_OK;
 }
 

@@ -40,7 +40,7 @@
 
 namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
-                 int logw = 16);
+                 int logw = 16, bool ot = false);
 int fused_prepare();
 uint32_t fused_items(int logN, size_t limbs, int logw = 16);
 }  // namespace ntt
@@ -353,7 +353,7 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
         p.jobs = s.d_desc + g.first;
         p.njobs = (uint32_t)g.count;
         p.total_items = g.items;
-        if ((rc = launch_fused(g.n, g.inv, g.v.eb, g.v.lazy, p, s.stream, 0, g.v.logw))) return rc;
+        if ((rc = launch_fused(g.n, g.inv, g.v.eb, g.v.lazy, p, s.stream, 0, g.v.logw, g.v.ot))) return rc;
         ++*fused;
     }
     return ck(hipEventRecord(s.done, s.stream), "hipEventRecord(slot)");

@@ -32,7 +32,7 @@
 
 namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
-                 int logw = 16);
+                 int logw = 16, bool ot = false);
 uint32_t fused_items(int logN, size_t limbs, int logw = 16);
 bool fused_supported(int logN);
 }  // namespace ntt

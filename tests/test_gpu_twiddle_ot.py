@@ -24,7 +24,8 @@ def _print(*a):
 
 @pytest.mark.parametrize("logN,limbs", [(10, 67), (11, 5), (12, 9), (13, 3), (14, 6), (15, 3), (16, 4)])
 @pytest.mark.parametrize("domain", [0, 1])
-def test_ot_matches_oracle(env, logN, limbs, domain):
+@pytest.mark.parametrize("arith", [1, 0])
+def test_ot_matches_oracle(env, logN, limbs, domain, arith):
     ntt_amd, gu, orc = env
     moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
     count = 2
@@ -33,8 +34,11 @@ def test_ot_matches_oracle(env, logN, limbs, domain):
     plan = ctx.plan(logN, moduli, domain)
     plan.set_engine(ntt_amd.ENGINE_FUSED)
     plan.set_twiddle_mode(ntt_amd.TWIDDLE_OT)
+    plan.set_fused_arith(arith)
+    for direction in (0, 1):
+        assert plan.fused_variant_full(count, direction) == (6, arith == 1, 16, True)
     A = gu.to_dev(x)
-    _print("case ot", logN, limbs, domain)
+    _print("case ot", logN, limbs, domain, arith)
     plan.forward(A, count)
     gu.sync()
     fwd = gu.to_host(A).reshape(x.shape)
@@ -62,6 +66,8 @@ def test_ot_canonical_moduli_and_table_mode_agree(env):
     plan = ctx.plan(logN, moduli, 1)
     plan.set_engine(ntt_amd.ENGINE_FUSED)
     a, b = gu.to_dev(x), gu.to_dev(x)
+    plan.set_twiddle_mode(ntt_amd.TWIDDLE_TABLE)
+    assert not plan.fused_variant_full(1, 0)[3]
     plan.forward(a)
     plan.set_twiddle_mode(ntt_amd.TWIDDLE_OT)
     _print("case ot canonical")
