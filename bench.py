@@ -681,7 +681,7 @@ def main():
 
     # the binding compute roof: the same butterfly code on register-resident operands
     inverse_dom = dom.split("<", 1)[1].split(",")[1].strip() == "true"   # second template argument: INV
-    lazy_dom = (lazy and engine_used != "fused") or (engine_used == "fused" and not inverse_dom and lzf)
+    lazy_dom = (lazy and engine_used != "fused") or (engine_used == "fused" and (lzi if inverse_dom else lzf))
     variant = (dev.BUTTERFLY_GS if inverse_dom else dev.BUTTERFLY_CT) + (1 if lazy_dom else 0)
     roof_bfly = dev.butterfly_roof(variant, stream)
     bfly_dom = B * bfly_per_transform / (kernel_ms[dom] * 1e-3)

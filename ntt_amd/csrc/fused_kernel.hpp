@@ -158,9 +158,12 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
     constexpr int LO = win_lo(n, INV, K, EB);
     constexpr int JLO = stage_jlo(n, INV, K, EB), JHI = stage_jhi(n, INV, K, EB);
     const uint32_t TB = (1u << (n - EB - LO)) + (opaque(u) >> LO);
+    // The inverse's last stage (bit n - 1, twiddle tw[1] for every word) is
+    // folded into the output scaling by run_item.
+    constexpr bool FOLDED_TOP = INV && K == nrounds(n, EB) - 1;
     static_for<EB>([&](auto JJ) __attribute__((always_inline)) {
         constexpr int j = INV ? decltype(JJ)::value : EB - 1 - decltype(JJ)::value;
-        if constexpr (j >= JLO && j <= JHI) {
+        if constexpr (j >= JLO && j <= JHI && !(FOLDED_TOP && j == EB - 1)) {
             // on-the-fly twiddles of this stage: index (TB << k) + c with
             // c < 2^k, k <= 6, splits as lo = (TB mod 2^(6-k)) << k + c and
             // hi = TB >> (6 - k); msb[hi] is one word per thread per stage
@@ -345,9 +348,19 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 #pragma unroll
     for (int x = 0; x < E; ++x) asm volatile("" : "+v"(v[x]));
     if (valid) {
-        if (INV && (c.flags & 1)) {
-#pragma unroll
-            for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], c.scale, c.q, c.q_inv);
+        if (INV) {
+            // the last stage (bit n - 1, twiddle w = itw[1]) folded with the
+            // output scaling: ((a + b) * scale, (a - b) * (scale w / R)) / R;
+            // a + b and a - b + 2q (q for canonical a, b) stay below 2^32
+            const uint32_t s2 = mont_mul(c.scale, tw[1], c.q, c.q_inv);
+            const uint32_t qq = LAZY ? 2 * c.q : c.q;
+            static_for<E / 2>([&](auto P) __attribute__((always_inline)) {
+                constexpr int x = decltype(P)::value, y = x | (E / 2);
+                constexpr int ra = REG<n, INV, NR - 1, x, EB>, rb = REG<n, INV, NR - 1, y, EB>;
+                const uint32_t a = v[ra], b = v[rb];
+                v[ra] = mont_mul(a + b, c.scale, c.q, c.q_inv);
+                v[rb] = mont_mul(a + qq - b, s2, c.q, c.q_inv);
+            });
         } else if (LAZY) {
             // forward values lie in [0, 4q), unscaled inverse ones in [0, 2q)
 #pragma unroll

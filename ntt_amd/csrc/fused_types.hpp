@@ -41,7 +41,7 @@ struct LimbConst {
     uint32_t q;
     uint32_t q_inv;
     uint32_t scale;    // Montgomery multiplier: forward input / inverse output
-    uint32_t flags;    // bit 0: apply scale
+    uint32_t flags;    // bit 0: apply scale (forward; the inverse always applies it)
 };
 
 // Operands of `period` limbs of size 2^n stored back to back at `data`.  The
