@@ -157,7 +157,13 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
     constexpr int E = 1 << EB;
     constexpr int LO = win_lo(n, INV, K, EB);
     constexpr int JLO = stage_jlo(n, INV, K, EB), JHI = stage_jhi(n, INV, K, EB);
-    const uint32_t TB = (1u << (n - EB - LO)) + (opaque(u) >> LO);
+    // In the forward's first round (window at the top bits, LO + EB == n)
+    // every thread of a limb has u < 2^LO, so TB = 1 and the stage twiddles
+    // are tw[1 .. 2^EB - 1] for the whole limb: compile-time offsets from the
+    // (wave-uniform) table base, read with scalar loads into SGPRs.  (The
+    // inverse's last round has the same property, but there the SGPR
+    // twiddles raised the lazy kernels' spills and measured slower.)
+    const uint32_t TB = (!INV && LO + EB == n) ? 1u : (1u << (n - EB - LO)) + (opaque(u) >> LO);
     // The inverse's last stage (bit n - 1, twiddle tw[1] for every word) is
     // folded into the output scaling by run_item.
     constexpr bool FOLDED_TOP = INV && K == nrounds(n, EB) - 1;
