@@ -58,8 +58,10 @@ int ntt_rt_fill_uniform(uint32_t* dst, const uint32_t* moduli_dev, size_t limbs,
  * the transform kernels' own butterfly code on register-resident operands
  * (no memory or LDS traffic).  variant 0: Cooley-Tukey, canonical; 1: CT,
  * lazy (q < 2^30); 2: Gentleman-Sande, canonical; 3: GS, lazy; 4: CT and
- * 5: GS on 64-bit words (q < 2^62, the ntt_plan64 butterflies).  Blocks until
- * done (about a millisecond). */
+ * 5: GS on 64-bit words (q < 2^62, the ntt_plan64 butterflies); 6: CT, lazy,
+ * with a precomputed-quotient (Shoup) product and 7: the same loop with the
+ * Montgomery product (a comparison of the two products; eight twiddles per
+ * thread used in turn).  Blocks until done (about a millisecond). */
 int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream);
 /* Plain streaming copy of `bytes` (multiple of 16, 16-byte aligned): reads and
  * writes each byte once; the reference kernel for bandwidth and for scaling

@@ -53,6 +53,55 @@ __global__ __launch_bounds__(ROOF_THREADS) void butterfly_roof_kernel(uint32_t* 
     out[t] = r;
 }
 
+// Cooley-Tukey on x, y in [0, 4q) with a precomputed-quotient (Shoup /
+// Harvey) product instead of Montgomery's: with w' = floor(w 2^32 / q),
+// v = y w - floor(y w' / 2^32) q lies in [0, 2q) for every 32-bit y.  The
+// product is a high multiply, a low multiply and a multiply-add (three VALU
+// instructions, as the Montgomery product), and every twiddle needs its
+// quotient word w' beside it.
+__device__ __forceinline__ void ct_butterfly_shoup_lazy(uint32_t& x, uint32_t& y, uint32_t w, uint32_t wq,
+                                                        uint32_t q) {
+    const uint32_t q2 = 2 * q;
+    const uint32_t u = min(x, x - q2);
+    const uint32_t v = y * w - __umulhi(y, wq) * q;
+    x = u + v;
+    y = u + q2 - v;
+}
+
+// Variants 6 (Shoup) and 7 (Montgomery) of the lazy Cooley-Tukey butterfly
+// with the same loop: eight twiddles per thread (with their quotient words
+// for variant 6) held in registers and used in turn, so that the two
+// products are compared on equal terms.
+template <int VARIANT>
+__global__ __launch_bounds__(ROOF_THREADS) void butterfly_roof_pair_kernel(uint32_t* out, uint32_t q, uint32_t q_inv,
+                                                                           uint32_t w0) {
+    uint32_t X[ROOF_CHAINS], Y[ROOF_CHAINS], W[8], WQ[8];
+    const uint32_t t = blockIdx.x * ROOF_THREADS + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < ROOF_CHAINS; ++i) {
+        X[i] = (t * 7u + (uint32_t)i) % q;
+        Y[i] = (t * 13u + 5u * (uint32_t)i) % q;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        W[k] = (w0 + t + 97u * (uint32_t)k) % q;
+        WQ[k] = (uint32_t)(((uint64_t)W[k] << 32) / q);
+    }
+    for (int it = 0; it < ROOF_ITERS; it += 8) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+#pragma unroll
+            for (int i = 0; i < ROOF_CHAINS; ++i) {
+                if (VARIANT == 6) ct_butterfly_shoup_lazy(X[i], Y[i], W[k], WQ[k], q);
+                if (VARIANT == 7) ntt::ct_butterfly_lazy(X[i], Y[i], W[k], q, q_inv);
+            }
+    }
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < ROOF_CHAINS; ++i) r ^= X[i] ^ Y[i];
+    out[t] = r;
+}
+
 // The same for 64-bit words (variant 4: Cooley-Tukey, 5: Gentleman-Sande).
 template <int VARIANT>
 __global__ __launch_bounds__(ROOF_THREADS) void butterfly_roof64_kernel(uint32_t* out, uint64_t q, uint64_t q_inv,
@@ -90,7 +139,7 @@ __global__ __launch_bounds__(256) void stream_copy_kernel(const uint4* __restric
 extern "C" {
 
 int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream) {
-    if (!butterflies_per_s || variant < 0 || variant > 5) return NTT_ERR_INVALID_ARGUMENT;
+    if (!butterflies_per_s || variant < 0 || variant > 7) return NTT_ERR_INVALID_ARGUMENT;
     int dev = 0, cus = 0;
     int rc = ck(hipGetDevice(&dev), "hipGetDevice");
     if (rc) return rc;
@@ -105,7 +154,9 @@ int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream) 
         variant == 0 ? butterfly_roof_kernel<0>
         : variant == 1 ? butterfly_roof_kernel<1>
         : variant == 2 ? butterfly_roof_kernel<2>
-                       : butterfly_roof_kernel<3>;
+        : variant == 3 ? butterfly_roof_kernel<3>
+        : variant == 6 ? butterfly_roof_pair_kernel<6>
+                       : butterfly_roof_pair_kernel<7>;
     const uint32_t q = 1073479681u;     // a prime < 2^30 with q = 1 mod 2^17
     uint32_t inv = 1;
     for (int i = 0; i < 5; ++i) inv *= 2u - q * inv;
@@ -114,7 +165,7 @@ int ntt_rt_butterfly_roof(int variant, double* butterflies_per_s, void* stream) 
     uint64_t inv64 = 1;
     for (int i = 0; i < 6; ++i) inv64 *= 2u - q64 * inv64;
     const uint64_t q_inv64 = 0u - inv64;
-    const bool w64 = variant >= 4;
+    const bool w64 = variant == 4 || variant == 5;
     void (*fn64)(uint32_t*, uint64_t, uint64_t, uint64_t) =
         variant == 4 ? butterfly_roof64_kernel<4> : butterfly_roof64_kernel<5>;
     auto launch = [&](uint32_t salt) {

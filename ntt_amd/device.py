@@ -195,6 +195,7 @@ def fill_uniform(buf, moduli_dev, limbs: int, logN: int, count: int, seed: int, 
 
 BUTTERFLY_CT, BUTTERFLY_CT_LAZY, BUTTERFLY_GS, BUTTERFLY_GS_LAZY = 0, 1, 2, 3
 BUTTERFLY_CT64, BUTTERFLY_GS64 = 4, 5
+BUTTERFLY_CT_LAZY_SHOUP, BUTTERFLY_CT_LAZY_MONT8 = 6, 7
 
 
 def butterfly_roof(variant: int = BUTTERFLY_CT, stream=None) -> float:
