@@ -85,6 +85,7 @@ struct ntt_batch {
     std::vector<FusedGroup> groups;
     std::vector<FusedJob> desc;                 // host copy of the device job table
     FusedJob* d_desc = nullptr;
+    uint32_t* d_counters = nullptr;             // two words per fused group (dynamic scheduling)
     std::vector<TwoPassGroup> tp_groups;
     std::vector<size_t> job_limb0;              // two-pass jobs: first entry in the per-limb arrays
     std::vector<uint32_t*> h_A;                 // per-limb operand pointers (host copy)
@@ -110,6 +111,7 @@ struct ntt_batch {
             if (e) (void)hipEventDestroy(e);
         if (fork) (void)hipEventDestroy(fork);
         if (d_desc) (void)hipFree(d_desc);
+        if (d_counters) (void)hipFree(d_counters);
         if (d_tp) (void)hipFree(d_tp);
     }
 };
@@ -135,7 +137,11 @@ int enqueue(ntt_batch* b, hipStream_t stream) {
                 p.jobs = b->d_desc + g.desc_begin;
                 p.njobs = (uint32_t)g.njobs;
                 p.total_items = g.items;
-                p.counter = nullptr;          // static round-robin: nothing to reset between replays
+                // dynamic scheduling: groups on the batch's streams share the
+                // CUs, so a group's grid is not resident all at once; the
+                // kernel resets its counters itself, so graph replays need no
+                // memset node
+                p.counter = b->d_counters + 2 * (size_t)u.group;
                 p.stagger = 0;
                 rc = launch_fused(g.n, g.inverse, g.v.eb, g.v.lazy, p, s, 0, g.v.logw, g.v.ot);
             } else {
@@ -221,6 +227,25 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         else
             tp_shape[std::make_tuple(j.logN, j.direction == NTT_INVERSE ? 1 : 0, j.domain, it->second->lazy ? 1 : 0)]
                 .push_back(i);
+    }
+
+    // a fused-capable group whose size the autotuner measured faster on the
+    // two-pass kernels (small groups: a few hundred limbs of 2^12..2^15 words
+    // fill too few workgroups of the persistent kernel) joins the two-pass path
+    for (auto it = by_shape.begin(); it != by_shape.end();) {
+        size_t total_limbs = 0;
+        for (size_t i : it->second) total_limbs += b->jobs[i].limbs;
+        TunedEntry t;
+        if (ctx->tuned_lookup(std::get<0>(it->first), total_limbs, &t) && t.engine == NTT_ENGINE_TWO_PASS) {
+            for (size_t i : it->second) {
+                const ntt_batch_job& j = b->jobs[i];
+                tp_shape[std::make_tuple(j.logN, j.direction == NTT_INVERSE ? 1 : 0, j.domain,
+                                         b->job_plan[i]->lazy ? 1 : 0)].push_back(i);
+            }
+            it = by_shape.erase(it);
+        } else {
+            ++it;
+        }
     }
 
     // descriptor table, one contiguous run per group
@@ -328,6 +353,9 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         if (hipMalloc(&b->d_desc, bytes) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
         if ((rc = ck(hipMemcpy(b->d_desc, b->desc.data(), bytes, hipMemcpyHostToDevice), "hipMemcpy(jobs)")))
             return rc;
+        const size_t cb = 2 * b->groups.size() * sizeof(uint32_t);
+        if (hipMalloc(&b->d_counters, cb) != hipSuccess) return NTT_ERR_OUT_OF_MEMORY;
+        if ((rc = ck(hipMemset(b->d_counters, 0, cb), "hipMemset(counters)"))) return rc;
     }
     *out = b.release();
     return NTT_OK;

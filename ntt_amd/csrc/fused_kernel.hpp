@@ -455,7 +455,17 @@ void fused_kernel(FusedParams p) {
             ++iter;
         }
         item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= p.total_items) break;
+        if (item >= p.total_items) {
+            // dynamic scheduling: every workgroup leaves after one claim past
+            // the end, so the last one out (counter[1] reaching the grid size)
+            // is the last to touch the counters and returns both to zero for
+            // the next launch (graph replays included)
+            if (p.counter && threadIdx.x == 0 && atomicAdd(p.counter + 1, 1u) == gridDim.x - 1) {
+                atomicExch(p.counter, 0u);
+                atomicExch(p.counter + 1, 0u);
+            }
+            break;
+        }
         FusedJob job = p.job0;
         if (p.jobs) {
             // last job with item_begin <= item

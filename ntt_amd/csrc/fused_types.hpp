@@ -88,7 +88,12 @@ struct FusedParams {
     const FusedJob* jobs;     // device array, item_begin ascending
     uint32_t njobs;
     uint32_t total_items;
-    uint32_t* counter;        // dynamic scheduling; null = static round-robin
+    // Dynamic scheduling: workgroups claim items from counter[0]; the last
+    // workgroup to leave resets counter[0] and counter[1] to zero.  Two words
+    // of device memory, zero before the first launch, one pair per launch in
+    // flight.  Null: static round-robin (item b + k * grid), which assumes the
+    // whole grid resident at once.
+    uint32_t* counter;
     // Start of workgroup b is delayed by (b % 4) * stagger sleeps of 64*127
     // cycles each, so that the load/store phases of different CUs interleave
     // with the compute phases of others instead of all hitting HBM at once.

@@ -123,6 +123,7 @@ struct Slot {
     hipEvent_t done = nullptr;
     FusedJob* h_desc = nullptr;     // pinned staging
     FusedJob* d_desc = nullptr;
+    uint32_t* d_counter = nullptr;  // dynamic scheduling of the slot's fused launches (two words)
     // per-limb arrays of the two-pass groups (operand, table, q, q_inv, scale),
     // pinned staging and device copy, SLOT_LIMB_BYTES bytes per limb
     char* h_limb = nullptr;
@@ -174,6 +175,7 @@ struct ntt_server {
             if (s.done) (void)hipEventDestroy(s.done);
             if (s.h_desc) (void)hipHostFree(s.h_desc);
             if (s.d_desc) (void)hipFree(s.d_desc);
+            if (s.d_counter) (void)hipFree(s.d_counter);
             if (s.h_limb) (void)hipHostFree(s.h_limb);
             if (s.d_limb) (void)hipFree(s.d_limb);
             if (s.h_fill) (void)hipHostFree(s.h_fill);
@@ -266,10 +268,14 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
     for (auto& kv : groups) {
         const int n = std::get<0>(kv.first);
         const bool inv = std::get<1>(kv.first) == NTT_INVERSE;
-        if (fused_supported(n)) {
+        size_t tot = 0;
+        for (size_t i : kv.second) tot += reqs[i]->r.limbs;
+        // the engine the autotuner measured faster for a launch of this size
+        // (fused when the size is untuned)
+        TunedEntry tuned;
+        const bool tuned_two_pass = S->ctx->tuned_lookup(n, tot, &tuned) && tuned.engine == NTT_ENGINE_TWO_PASS;
+        if (fused_supported(n) && !tuned_two_pass) {
             const size_t first = nd;
-            size_t tot = 0;
-            for (size_t i : kv.second) tot += reqs[i]->r.limbs;
             const FusedVariant v = fused_variant_for(S->ctx, n, tot, inv, std::get<2>(kv.first) != 0);
             uint64_t items = 0;
             for (size_t i : kv.second) {
@@ -353,6 +359,10 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
         p.jobs = s.d_desc + g.first;
         p.njobs = (uint32_t)g.count;
         p.total_items = g.items;
+        // the slots' launches share the CUs: items are claimed dynamically; the
+        // slot's launches run in order on its stream, and each kernel resets
+        // the counter pair before the next one starts
+        p.counter = s.d_counter;
         if ((rc = launch_fused(g.n, g.inv, g.v.eb, g.v.lazy, p, s.stream, 0, g.v.logw, g.v.ot))) return rc;
         ++*fused;
     }
@@ -503,6 +513,11 @@ int ntt_server_create(ntt_context* ctx, const ntt_server_config* cfg, ntt_server
             s.d_desc = nullptr;
             return NTT_ERR_OUT_OF_MEMORY;
         }
+        if (hipMalloc(&s.d_counter, 2 * sizeof(uint32_t)) != hipSuccess) {
+            s.d_counter = nullptr;
+            return NTT_ERR_OUT_OF_MEMORY;
+        }
+        if ((rc = ck(hipMemset(s.d_counter, 0, 2 * sizeof(uint32_t)), "hipMemset(counter)"))) return rc;
         if ((rc = ck(hipHostMalloc((void**)&s.h_fill, cfg->max_jobs_per_launch * sizeof(FillJob), hipHostMallocDefault),
                      "hipHostMalloc")))
             return rc;
