@@ -258,6 +258,13 @@ def bench_served(ntt_amd, requests: int = 20000, fractions=(0.25, 0.5, 0.75, 0.9
     return out
 
 
+def fused_kernel_name(logN: int, inverse: bool, eb: int, lazy: bool, logw: int, ot: bool) -> str:
+    """Symbol of the fused kernel a plan launch runs, with its template
+    arguments as the kernel trace prints them."""
+    tf = lambda b: "true" if b else "false"
+    return f"fused_kernel<{logN}, {tf(inverse)}, {tf(lazy)}, {eb}, false, {tf(ot)}, {logw}>"
+
+
 def _timed(dev, stream, fn, reps):
     e0, e1 = dev.Event(), dev.Event()
     fn()
@@ -395,6 +402,22 @@ def bench_c1_plain(ntt_amd, dev, stream, traffic_path: str, logN: int = 16, limb
         single[f"single_{name}_ms_reference_style"] = round(
             _timed(dev, stream, lambda: (one.copy_from(src, stream=stream), fn()), 30), 4)
         single[f"single_{name}_ms_kernel_only"] = round(_timed(dev, stream, fn, 30), 4)
+    # the same pool through a plain-domain plan (the library's own entry
+    # point: cached Montgomery-form tables, fused kernels where they apply)
+    ctx = ntt_amd.Context(0)
+    plan = ctx.plan(logN, moduli, ntt_amd.PLAIN)
+    pf = _timed(dev, stream, lambda: plan.forward(pool, count, stream), reps)
+    pi = _timed(dev, stream, lambda: plan.inverse(pool, count, stream), reps)
+    pok = bool(np.array_equal(before, pool.numpy(np.uint32, nbytes=4 * words)))
+    pv = [plan.fused_variant_full(count, d) for d in (ntt_amd.FORWARD, ntt_amd.INVERSE)]
+    peng = {1: "two_pass", 2: "fused"}.get(plan.engine(count), "?")
+    plan_rec = {"engine": peng, "forward_ms": round(pf, 4), "inverse_ms": round(pi, 4),
+                "gbfly_s": round(2 * bfly / ((pf + pi) * 1e-3) / 1e9, 1),
+                "hbm_frac": round(2 * 8 * words * count / ((pf + pi) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                "kernels": {"forward": fused_kernel_name(logN, False, *pv[0]) if peng == "fused" else None,
+                            "inverse": fused_kernel_name(logN, True, *pv[1]) if peng == "fused" else None},
+                "roundtrip_ok": pok}
+    plan.close()
     step = fwd + inv
     out = {"workload": f"plain variant, N=2^{logN}, {limbs} limbs, {count} operands as one drop-in launch of {L} limbs "
                        f"with per-limb normal-domain tables ({4 * words * count / 2**30:.2f} GiB operands), seed 0",
@@ -409,7 +432,7 @@ def bench_c1_plain(ntt_amd, dev, stream, traffic_path: str, logN: int = 16, limb
                         "frac": round(8 * words * count / (max(fwd, inv) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                         "scope": "whole two-pass transform (both kernels) per direction",
                         "bytes_per_launch": 8 * words * count},
-           **single, "roundtrip_ok": ok}
+           **single, "roundtrip_ok": ok, "plan": plan_rec}
     for b in (tw, itw, pool, one, src):
         b.free()
     return out
@@ -642,9 +665,8 @@ def main():
     ebi, lzi, lwi, oti = plan.fused_variant_full(B, ntt_amd.INVERSE)
     lazy = all(q < (1 << 30) for q, _ in moduli)   # two-pass kernels then use lazy reduction
     if engine_used == "fused":
-        tf = lambda b: "true" if b else "false"
-        kern = {f"fused_kernel<{logN}, false, {tf(lzf)}, {ebf}, false, {tf(otf)}, {lwf}>": lambda: plan.forward(pool, B, stream),
-                f"fused_kernel<{logN}, true, {tf(lzi)}, {ebi}, false, {tf(oti)}, {lwi}>": lambda: plan.inverse(pool, B, stream)}
+        kern = {fused_kernel_name(logN, False, ebf, lzf, lwf, otf): lambda: plan.forward(pool, B, stream),
+                fused_kernel_name(logN, True, ebi, lzi, lwi, oti): lambda: plan.inverse(pool, B, stream)}
     else:
         mode = 2 if lazy else 0
         c = cfg.astuple()

@@ -57,7 +57,7 @@ int fused_prepare() {
     static std::once_flag attr_once;
     static hipError_t attr_err = hipSuccess;
     std::call_once(attr_once, [] {
-        const int lds = (int)(FUSED_LDS_WORDS * sizeof(uint32_t));
+        const int lds = (int)fused_lds_bytes(16);
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
             for (int d = 0; d < 8 && attr_err == hipSuccess; ++d) {
                 FusedFn f = fused_fn(m, d & 1, 6 + ((d >> 1) & 1), d >= 4);
@@ -72,7 +72,7 @@ int fused_prepare() {
                     attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds);
             }
-        const int lds15 = (int)(fused_lds_words(15) * sizeof(uint32_t));
+        const int lds15 = (int)fused_lds_bytes(15);
         for (int m = FUSED_MIN_LOG_N; m <= 15 && attr_err == hipSuccess; ++m)
             for (int d = 0; d < 8 && attr_err == hipSuccess; ++d) {
                 FusedFn f = fused_fn(m, d & 1, 6 + ((d >> 1) & 1), d >= 4, 15);
@@ -97,7 +97,7 @@ static int launch_fused_fn(FusedFn fn, int eb, const FusedParams& p, hipStream_t
                            int logw = 16) {
     int rc = fused_prepare();
     if (rc) return rc;
-    const size_t lds = fused_lds_words(logw) * sizeof(uint32_t);
+    const size_t lds = fused_lds_bytes(logw);
     unsigned grid = (unsigned)device_cus() << (16 - logw);
     if (grid_limit && grid_limit < grid) grid = grid_limit;
     if (p.total_items < grid) grid = p.total_items;

@@ -151,9 +151,12 @@ inline constexpr uint32_t LDS_OFF = lds_pad<EB>(compress<S>((uint32_t)X << LO));
 // its own twiddles), each twiddle is formed on the fly from the table's first
 // 64 words and its msb appendix (FusedJob tables, fused_types.hpp) instead of
 // being read from the table's upper levels.
-template <int n, bool INV, bool LAZY, int K, int EB, bool OT = false>
-__device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict__ tw, uint32_t u, uint32_t q,
-                                       uint32_t q_inv) {
+// TL: the stage twiddles (and the on-the-fly factors) come from the staged
+// LDS copy twl (tw[0 .. FUSED_TWL_TW - 1], then the msb appendix) instead of
+// the table in HBM.
+template <int n, bool INV, bool LAZY, int K, int EB, bool OT = false, bool TL = false>
+__device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict__ tw, const uint32_t* twl,
+                                       uint32_t u, uint32_t q, uint32_t q_inv) {
     constexpr int E = 1 << EB;
     constexpr int LO = win_lo(n, INV, K, EB);
     constexpr int JLO = stage_jlo(n, INV, K, EB), JHI = stage_jhi(n, INV, K, EB);
@@ -177,7 +180,7 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
             constexpr bool OTS = OT && LO == 0 && k <= OT_LSB_BITS;
             uint32_t ot_msb = 0, ot_lo = 0;
             if constexpr (OTS) {
-                ot_msb = tw[(1u << n) + (TB >> (OT_LSB_BITS - k))];
+                ot_msb = TL ? twl[FUSED_TWL_TW + (TB >> (OT_LSB_BITS - k))] : tw[(1u << n) + (TB >> (OT_LSB_BITS - k))];
                 ot_lo = (TB & ((1u << (OT_LSB_BITS - k)) - 1)) << k;
             }
             static_for<E / 2>([&](auto P) __attribute__((always_inline)) {
@@ -187,7 +190,10 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
                 constexpr int a = REG<n, INV, K, x, EB>, b = REG<n, INV, K, x | (1 << j), EB>;
                 uint32_t w;
                 if constexpr (OTS)
-                    w = mont_mul(tw[ot_lo + (uint32_t)(x >> (j + 1))], ot_msb, q, q_inv);
+                    w = mont_mul(TL ? twl[ot_lo + (uint32_t)(x >> (j + 1))] : tw[ot_lo + (uint32_t)(x >> (j + 1))],
+                                 ot_msb, q, q_inv);
+                else if constexpr (TL)
+                    w = twl[(TB << (EB - 1 - j)) + (uint32_t)(x >> (j + 1))];
                 else   // 32-bit byte offset: SGPR table base plus a VGPR offset per load
                     w = *(gcu32*)((gcu8*)tw + (((TB << (EB - 1 - j)) + (uint32_t)(x >> (j + 1))) << 2));
                 if (LAZY) {
@@ -276,8 +282,12 @@ __device__ __forceinline__ FusedJob uniform_job(const FusedJob& j) {
     return u;
 }
 
-template <int n, bool INV, bool LAZY, int EB, bool MD, bool OT>
-__device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds) {
+// TWL: one limb of 2^16 words per workgroup with on-the-fly twiddles; the
+// twiddle words the rounds read are staged into LDS after the exchange
+// buffer whenever the limb's table differs from the staged one (*staged).
+template <int n, bool INV, bool LAZY, int EB, bool MD, bool OT, bool TWL = false>
+__device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds,
+                                         const uint32_t** staged) {
     constexpr int E = 1 << EB;
     constexpr int G_BITS = n - EB;
     constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 1 - pad_shift(EB)));
@@ -309,6 +319,18 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         c.flags = __builtin_amdgcn_readfirstlane(c.flags);
     }
     uint32_t* L = lds + slot * STRIDE;
+    uint32_t* const twl = lds + FUSED_LDS_WORDS;
+    if constexpr (TWL) {
+        static_assert(n == 16 && OT, "staged twiddles: one 2^16-word limb per workgroup, on-the-fly twiddles");
+        if ((const uint32_t*)tw != *staged) {   // uniform: the table base is in SGPRs
+            __syncthreads();   // the previous item's rounds have read the staged words
+            for (uint32_t i = threadIdx.x; i < FUSED_TWL_TW; i += blockDim.x) twl[i] = tw[i];
+            for (uint32_t i = threadIdx.x; i < FUSED_TWL_WORDS - FUSED_TWL_TW; i += blockDim.x)
+                twl[FUSED_TWL_TW + i] = tw[(1u << n) + i];
+            __syncthreads();
+            *staged = (const uint32_t*)tw;
+        }
+    }
 
     uint32_t v[E];
     // ---- load (round 0 layout) ----
@@ -339,12 +361,14 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         for (int x = 0; x < E; ++x) v[x] = 0;
     }
 
-    stages<n, INV, LAZY, 0, EB, OT>(v, tw, u, c.q, c.q_inv);
+    // staged twiddles: every round of the inverse, the forward's rounds after
+    // the first (whose twiddles tw[1 .. 63] are scalar loads)
+    stages<n, INV, LAZY, 0, EB, OT, TWL && INV>(v, tw, twl, u, c.q, c.q_inv);
     exchange<n, INV, 0, EB>(v, L, u, valid);
-    stages<n, INV, LAZY, 1, EB, OT>(v, tw, u, c.q, c.q_inv);
+    stages<n, INV, LAZY, 1, EB, OT, TWL>(v, tw, twl, u, c.q, c.q_inv);
     if constexpr (NR == 3) {
         exchange<n, INV, 1, EB>(v, L, u, valid);
-        stages<n, INV, LAZY, 2, EB, OT>(v, tw, u, c.q, c.q_inv);
+        stages<n, INV, LAZY, 2, EB, OT, TWL>(v, tw, twl, u, c.q, c.q_inv);
     }
 
     // ---- store (last round layout) ----
@@ -442,6 +466,8 @@ void fused_kernel(FusedParams p) {
     extern __shared__ uint32_t lds[];
     __shared__ uint32_t s_next;
     uint32_t iter = 0;
+    constexpr bool TWL = OT && n == 16 && LOGW == 16 && EB == 6;
+    const uint32_t* staged = nullptr;   // table whose words the LDS holds (TWL)
     for (uint32_t s = 0, ns = (blockIdx.x & 3u) * p.stagger; s < ns; ++s) __builtin_amdgcn_s_sleep(127);
     for (;;) {
         uint32_t item;
@@ -477,7 +503,7 @@ void fused_kernel(FusedParams p) {
             job = p.jobs[lo];
         }
         job = fused::uniform_job(job);
-        fused::run_item<n, INV, LAZY, EB, MD, OT>(job, (item - job.item_begin) << (LOGW - n), lds);
+        fused::run_item<n, INV, LAZY, EB, MD, OT, TWL>(job, (item - job.item_begin) << (LOGW - n), lds, &staged);
     }
 }
 

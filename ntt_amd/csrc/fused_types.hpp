@@ -11,6 +11,15 @@ namespace ntt {
 // padding word per 32.
 constexpr uint32_t FUSED_LDS_WORDS = 32768 + 1024;   // logw = 16
 constexpr uint32_t fused_lds_words(int logw) { return (1u << (logw - 1)) + (1u << (logw - 6)); }
+// After the exchange buffer of a 2^16-word workgroup, the on-the-fly
+// twiddle kernels for N = 2^16 stage the twiddle words their later rounds
+// read: tw[0 .. 2047] and the msb appendix msb[0 .. 1023] (12 KiB; the CU
+// has 160 KiB of LDS, the exchange buffer takes 132 KiB).
+constexpr uint32_t FUSED_TWL_TW = 2048, FUSED_TWL_WORDS = 2048 + 1024;
+// Dynamic LDS bytes of a fused launch with 2^logw words per workgroup.
+constexpr uint32_t fused_lds_bytes(int logw) {
+    return (fused_lds_words(logw) + (logw == 16 ? FUSED_TWL_WORDS : 0u)) * 4u;
+}
 // Default window widths (7: 512 threads x 128 words per CU; 6: 1024 threads
 // x 64 words) per direction, the measured faster ones at N = 2^16 on MI355X
 // (with lazy arithmetic in both directions where every q < 2^30); the
