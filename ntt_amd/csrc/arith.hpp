@@ -108,6 +108,17 @@ __device__ __forceinline__ void ct_butterfly_lazy(uint32_t& x, uint32_t& y, uint
     y = u + q2 - v;
 }
 
+// The last Cooley-Tukey stage of a lazy transform, x, y in [0, 4q): the
+// canonical (x + wy, x - wy) in [0, q), which saves canonicalising both
+// outputs after a lazy butterfly (12 instructions instead of 8 + 8).
+__device__ __forceinline__ void ct_butterfly_lazy_last(uint32_t& x, uint32_t& y, uint32_t w, uint32_t q,
+                                                       uint32_t q_inv) {
+    const uint32_t u = canon_4q(x, q);
+    const uint32_t v = mont_mul(y, w, q, q_inv);
+    x = add_mod(u, v, q);
+    y = sub_mod(u, v, q);
+}
+
 // Gentleman-Sande on x, y in [0, 2q): (x + y mod 2q, (x - y + 2q) w), both in [0, 2q).
 __device__ __forceinline__ void gs_butterfly_lazy(uint32_t& x, uint32_t& y, uint32_t w, uint32_t q,
                                                   uint32_t q_inv) {

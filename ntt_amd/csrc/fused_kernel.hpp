@@ -170,6 +170,9 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
     // The inverse's last stage (bit n - 1, twiddle tw[1] for every word) is
     // folded into the output scaling by run_item.
     constexpr bool FOLDED_TOP = INV && K == nrounds(n, EB) - 1;
+    // The forward's last stage (bit 0, in the last round) leaves canonical
+    // values when the arithmetic is lazy.
+    constexpr bool LAST_CANON = !INV && K == nrounds(n, EB) - 1;
     static_for<EB>([&](auto JJ) __attribute__((always_inline)) {
         constexpr int j = INV ? decltype(JJ)::value : EB - 1 - decltype(JJ)::value;
         if constexpr (j >= JLO && j <= JHI && !(FOLDED_TOP && j == EB - 1)) {
@@ -199,6 +202,8 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
                 if (LAZY) {
                     if (INV)
                         gs_butterfly_lazy(v[a], v[b], w, q, q_inv);
+                    else if (LAST_CANON && j == 0)
+                        ct_butterfly_lazy_last(v[a], v[b], w, q, q_inv);
                     else
                         ct_butterfly_lazy(v[a], v[b], w, q, q_inv);
                 } else if (INV) {
@@ -391,11 +396,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
                 v[ra] = mont_mul(a + b, c.scale, c.q, c.q_inv);
                 v[rb] = mont_mul(a + qq - b, s2, c.q, c.q_inv);
             });
-        } else if (LAZY) {
-            // forward values lie in [0, 4q), unscaled inverse ones in [0, 2q)
-#pragma unroll
-            for (int x = 0; x < E; ++x) v[x] = canon_4q(v[x], c.q);
-        }
+        }   // forward values are canonical: the last stage's lazy butterflies leave [0, q)
         if constexpr (MD) {
             // mod-down: (sub - NTT) * post, read at the positions stored below;
             // limbs outside [sub_start, sub_end) subtract from 0
