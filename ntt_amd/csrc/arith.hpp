@@ -17,6 +17,7 @@ namespace ntt {
 typedef __attribute__((address_space(1))) uint32_t gu32;
 typedef __attribute__((address_space(1))) const uint32_t gcu32;
 typedef __attribute__((address_space(1))) const unsigned char gcu8;
+typedef __attribute__((address_space(1))) unsigned char gu8;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) u32x4 gu4;
 typedef __attribute__((address_space(1))) const u32x4 gcu4;

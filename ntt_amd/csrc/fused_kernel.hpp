@@ -256,6 +256,14 @@ __device__ __forceinline__ void exchange(uint32_t (&v)[1 << EB], uint32_t* L, ui
 // or merged with one from the kernel arguments, lands in VGPRs.  Moving the
 // fields to SGPRs keeps pointers out of VGPRs (global accesses then take an
 // SGPR base plus a 32-bit VGPR offset) and frees registers for the words.
+// Orders the LDS accesses of one wavefront (they complete in order; this
+// keeps the compiler from moving them across the point).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class T>
 __device__ __forceinline__ T* uniform_ptr(T* p) {
     const uint64_t v = reinterpret_cast<uint64_t>(p);
@@ -298,6 +306,9 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 1 - pad_shift(EB)));
     constexpr int NR = nrounds(n, EB);
     constexpr int LO0 = win_lo(n, INV, 0, EB), LOL = win_lo(n, INV, NR - 1, EB);
+    // forward stores through an in-wave transpose (one limb per wave, 64
+    // words per thread, the last window at the low bits)
+    constexpr bool XST = !INV && !MD && EB == 6 && G_BITS >= 6 && LOL == 0;
     const uint32_t t = threadIdx.x;
     uint32_t slot = t >> G_BITS;
     uint32_t u = t & ((1u << G_BITS) - 1);
@@ -434,7 +445,46 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
             }
         }
         gu32* A = (gu32*)(job.data + (dlimb << n));
-        if (LOL == 0) {
+        if constexpr (XST) {
+            // The wave's 64 threads hold 4096 consecutive words, lane l the 64
+            // words (16 chunks of 16 B) at 64 l.  Stored as they are, each
+            // 16-byte store would put its lanes 256 B apart; instead the wave
+            // transposes its block through its own 8 KiB of the (now idle)
+            // exchange buffer, half of the chunks at a time: in phase h every
+            // lane writes its chunks 8h .. 8h + 7 to a 128-byte row (slot s of
+            // lane l at s ^ ((l >> 1) & 7), distinct banks for the row writes
+            // and the column reads), then store k writes words 256 k + 4 l from
+            // the lanes l whose chunk l & 15 is in this phase: 128-byte lines
+            // written whole.  Only wave-local synchronisation is needed.
+            // (opaque: the addresses below are computed per item, not hoisted
+            // out of the persistent loop and kept live)
+            const uint32_t lane = opaque(threadIdx.x & 63u);
+            u32x4* const Wl = (u32x4*)(lds + ((threadIdx.x >> 6) << 11));
+            // byte offset of store 0 from the (wave-uniform) limb base: an SGPR
+            // base plus one 32-bit VGPR offset per store
+            const uint32_t so = ((((opaque(u) >> 6) << 12) + 4 * lane) << 2);
+            gu8* const Ab = (gu8*)A;
+            const uint32_t wsw = (lane >> 1) & 7u;
+            static_for<2>([&](auto H) __attribute__((always_inline)) {
+                constexpr uint32_t h = decltype(H)::value;
+                static_for<8>([&](auto S) __attribute__((always_inline)) {
+                    constexpr uint32_t ss = decltype(S)::value, cc = 8 * h + ss;
+                    Wl[(lane << 3) + (ss ^ wsw)] =
+                        u32x4{v[REG<n, INV, NR - 1, 4 * cc, EB>], v[REG<n, INV, NR - 1, 4 * cc + 1, EB>],
+                              v[REG<n, INV, NR - 1, 4 * cc + 2, EB>], v[REG<n, INV, NR - 1, 4 * cc + 3, EB>]};
+                });
+                wave_lds_sync();
+                if (((lane >> 3) & 1u) == h) {   // lane & 15 in [8h, 8h + 8)
+                    const uint32_t sl = lane & 7u;
+                    static_for<16>([&](auto K) __attribute__((always_inline)) {
+                        constexpr uint32_t kk = decltype(K)::value;
+                        const uint32_t row = 4 * kk + (lane >> 4);   // the source lane
+                        *(gu4*)(Ab + (so + 1024u * kk)) = Wl[(row << 3) + (sl ^ ((row >> 1) & 7u))];
+                    });
+                }
+                wave_lds_sync();
+            });
+        } else if (LOL == 0) {
             gu4* dst = (gu4*)(A + ubase<LOL, EB>(opaque(u)));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
                 constexpr int x4 = decltype(X4)::value;
@@ -449,6 +499,8 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
             });
         }
     }
+    // the next item's first exchange writes the LDS the transposes read
+    if constexpr (XST) __syncthreads();
 }
 
 }  // namespace fused
@@ -464,7 +516,7 @@ template <int n, bool INV, bool LAZY, int EB, bool MD = false, bool OT = false, 
 __global__ __launch_bounds__(1 << (LOGW - EB)) __attribute__((amdgpu_waves_per_eu(1 << (8 - EB))))
 void fused_kernel(FusedParams p) {
     static_assert(n <= LOGW && LOGW >= 15 && LOGW <= 16, "a workgroup holds whole limbs of at most 2^LOGW words");
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     __shared__ uint32_t s_next;
     uint32_t iter = 0;
     constexpr bool TWL = OT && n == 16 && LOGW == 16 && EB == 6;
