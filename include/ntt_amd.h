@@ -226,7 +226,9 @@ int ntt_plan_set_fused_words(ntt_plan* plan, int log2_words);
  * reads 2^10 + N/64 table words per limb in place of N, at one Montgomery
  * multiply per twiddle of that round; the shape is 1024 threads x 64 words,
  * with the plan's arithmetic (lazy when every q < 2^30).  Two-pass launches
- * and the mod-down form read the table as before. */
+ * and the mod-down form read the table as before.  NTT_TWIDDLE_AUTO (the
+ * default) is the tuned choice per direction, untuned on the fly for the
+ * inverse and table twiddles for the forward. */
 int ntt_plan_set_twiddle_mode(ntt_plan* plan, int mode);
 /* Arithmetic of the fused kernels, both directions: 1 = lazy reduction (used
  * only when every q < 2^30), 0 = canonical, -1 = the tuned choice (default

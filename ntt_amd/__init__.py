@@ -800,7 +800,8 @@ class Plan:
                "ntt_plan_set_constants")
 
     def set_twiddle_mode(self, mode: int):
-        """TWIDDLE_AUTO (default: the tuned choice, else on the fly), TWIDDLE_TABLE or TWIDDLE_OT."""
+        """TWIDDLE_AUTO (default: the tuned choice; untuned, on the fly for the inverse and table twiddles
+        for the forward), TWIDDLE_TABLE or TWIDDLE_OT."""
         _check(_lib.ntt_plan_set_twiddle_mode(self.handle, mode), "ntt_plan_set_twiddle_mode")
 
     def set_fused_arith(self, lazy: int):

@@ -49,7 +49,7 @@ struct TunedEntry {
     bool fused_lazy = true;             // fused forward arithmetic: lazy (when every q < 2^30)
     bool fused_lazy_inv = true;         // fused inverse arithmetic: lazy (when every q < 2^30)
     int fused_logw_fwd = 16, fused_logw_inv = 16;   // fused words per workgroup (log2) per direction
-    bool fused_ot_fwd = true, fused_ot_inv = true;  // fused on-the-fly twiddles per direction
+    bool fused_ot_fwd = false, fused_ot_inv = true;  // fused on-the-fly twiddles per direction
     int stagger = 0;                    // fused engine: start stagger (FusedParams::stagger)
     size_t chunk = 0;          // operands per chunk, 0 = whole launch
     double fwd_ms = 0, inv_ms = 0;

@@ -179,7 +179,10 @@ static FusedVariant variant_of(const TunedEntry* t, int logN, bool inverse, bool
     v.lazy = (t ? (inverse ? t->fused_lazy_inv : t->fused_lazy) : true) && lazy_capable;
     v.logw = t ? (inverse ? t->fused_logw_inv : t->fused_logw_fwd) : 16;
     if (logN > 15) v.logw = 16;
-    v.ot = t ? (inverse ? t->fused_ot_inv : t->fused_ot_fwd) : true;
+    // untuned: on-the-fly twiddles for the inverse (its staged twiddle words
+    // made it the faster one), table twiddles for the forward (equal or
+    // faster since its stores go through the in-wave transpose)
+    v.ot = t ? (inverse ? t->fused_ot_inv : t->fused_ot_fwd) : inverse;
     if (v.ot) {   // the on-the-fly twiddle kernels: 1024 x 64 words per CU
         v.eb = 6;
         v.logw = 16;
