@@ -42,8 +42,7 @@ FusedFn fused_fn_w15(int n, bool inv, int eb, bool lazy);   // fused_w15.hip
 // Window width of the on-the-fly twiddle kernels.
 constexpr int OT_EB = 6;
 
-// Window width of the mod-down kernels: 7 with lazy arithmetic, 6 canonical.
-static int md_eb(bool lazy) { return lazy ? 7 : 6; }
+int fused_md_eb(int n, bool lazy);                  // fused_md.hip: window width of fused_fn_md(n, lazy)
 
 static FusedFn fused_fn(int n, bool inv, int eb, bool lazy, int logw = 16) {
     if (logw == 15) return fused_fn_w15(n, inv, eb, lazy);
@@ -177,7 +176,7 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
         FusedFn fn = fused_fn_md(P->logN, lz);
         if (!fn) return NTT_ERR_CONFIG;
         if (p.total_items == 0) return NTT_OK;
-        return launch_fused_fn(fn, md_eb(lz), p, stream, 0);
+        return launch_fused_fn(fn, fused_md_eb(P->logN, lz), p, stream, 0);
     }
     // lazy reduction needs every q < 2^30
     return launch_fused(P->logN, inverse, v.eb, v.lazy && P->lazy, p, stream, 0, w15 ? 15 : 16, false);

@@ -306,9 +306,9 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 1 - pad_shift(EB)));
     constexpr int NR = nrounds(n, EB);
     constexpr int LO0 = win_lo(n, INV, 0, EB), LOL = win_lo(n, INV, NR - 1, EB);
-    // forward stores through an in-wave transpose (one limb per wave, 64
-    // words per thread, the last window at the low bits)
-    constexpr bool XST = !INV && !MD && EB == 6 && G_BITS >= 6 && LOL == 0;
+    // forward stores (the mod-down form's too) through an in-wave transpose
+    // (one limb per wave, 64 words per thread, the last window at the low bits)
+    constexpr bool XST = !INV && EB == 6 && G_BITS >= 6 && LOL == 0;
     const uint32_t t = threadIdx.x;
     uint32_t slot = t >> G_BITS;
     uint32_t u = t & ((1u << G_BITS) - 1);
@@ -409,8 +409,9 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
             });
         }   // forward values are canonical: the last stage's lazy butterflies leave [0, q)
         if constexpr (MD) {
-            // mod-down: (sub - NTT) * post, read at the positions stored below;
-            // limbs outside [sub_start, sub_end) subtract from 0
+            // mod-down: (sub - NTT) * post, sub read at the positions of the
+            // words in registers (before the transposed store, if any); limbs
+            // outside [sub_start, sub_end) subtract from 0
             uint32_t f = job.post[per];
             if constexpr (G_BITS >= 6) f = __builtin_amdgcn_readfirstlane(f);
             const bool has_sub = per >= job.sub_start && per < job.sub_end;
