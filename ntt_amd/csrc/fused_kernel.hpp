@@ -156,7 +156,7 @@ inline constexpr uint32_t LDS_OFF = lds_pad<EB>(compress<S>((uint32_t)X << LO));
 // the table in HBM.
 template <int n, bool INV, bool LAZY, int K, int EB, bool OT = false, bool TL = false>
 __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict__ tw, const uint32_t* twl,
-                                       uint32_t u, uint32_t q, uint32_t q_inv) {
+                                       uint32_t u, uint32_t q, uint32_t q_inv, uint32_t scale = 0) {
     constexpr int E = 1 << EB;
     constexpr int LO = win_lo(n, INV, K, EB);
     constexpr int JLO = stage_jlo(n, INV, K, EB), JHI = stage_jhi(n, INV, K, EB);
@@ -173,9 +173,29 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
     // The forward's last stage (bit 0, in the last round) leaves canonical
     // values when the arithmetic is lazy.
     constexpr bool LAST_CANON = !INV && K == nrounds(n, EB) - 1;
+    // The lazy forward's first stage (bit n - 1, twiddle tw[1] for every
+    // pair) applies the input scale s as well: x' = x s / R and
+    // y' = y (s w) / R, both lazy in [0, 2q), then (x' + y', x' + 2q - y')
+    // in [0, 4q): two Montgomery products per pair instead of three.
+    constexpr bool FOLD_SCALE = LAZY && !INV && K == 0;
     static_for<EB>([&](auto JJ) __attribute__((always_inline)) {
         constexpr int j = INV ? decltype(JJ)::value : EB - 1 - decltype(JJ)::value;
-        if constexpr (j >= JLO && j <= JHI && !(FOLDED_TOP && j == EB - 1)) {
+        if constexpr (FOLD_SCALE && j == EB - 1) {
+            static_assert(LO + EB == n, "the forward's first round holds the top bits");
+            const uint32_t sw = mont_mul(scale, *(gcu32*)((gcu8*)tw + 4), q, q_inv);
+            const uint32_t q2 = 2 * q;
+            static_for<E / 2>([&](auto P) __attribute__((always_inline)) {
+                constexpr int x = decltype(P)::value;
+                constexpr int a = REG<n, INV, K, x, EB>, b = REG<n, INV, K, x | (1 << j), EB>;
+                // opaque: otherwise the compiler widens every loaded word
+                // into a zero-extended register pair for the 64-bit products
+                // (twice the registers for the words in flight, and spills)
+                const uint32_t xs = mont_mul_lazy(opaque(v[a]), scale, q, q_inv);
+                const uint32_t ys = mont_mul_lazy(opaque(v[b]), sw, q, q_inv);
+                v[a] = xs + ys;
+                v[b] = xs + q2 - ys;
+            });
+        } else if constexpr (j >= JLO && j <= JHI && !(FOLDED_TOP && j == EB - 1)) {
             // on-the-fly twiddles of this stage: index (TB << k) + c with
             // c < 2^k, k <= 6, splits as lo = (TB mod 2^(6-k)) << k + c and
             // hi = TB >> (6 - k); msb[hi] is one word per thread per stage
@@ -368,7 +388,8 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
                 v[REG<n, INV, 0, x, EB>] = Ab[x << LO0];
             });
         }
-        if (!INV && (c.flags & 1)) {
+        // the input scale (the lazy forward applies it in its first stage)
+        if (!INV && !LAZY && (c.flags & 1)) {
 #pragma unroll
             for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], c.scale, c.q, c.q_inv);
         }
@@ -379,7 +400,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 
     // staged twiddles: every round of the inverse, the forward's rounds after
     // the first (whose twiddles tw[1 .. 63] are scalar loads)
-    stages<n, INV, LAZY, 0, EB, OT, TWL && INV>(v, tw, twl, u, c.q, c.q_inv);
+    stages<n, INV, LAZY, 0, EB, OT, TWL && INV>(v, tw, twl, u, c.q, c.q_inv, c.scale);
     exchange<n, INV, 0, EB>(v, L, u, valid);
     stages<n, INV, LAZY, 1, EB, OT, TWL>(v, tw, twl, u, c.q, c.q_inv);
     if constexpr (NR == 3) {

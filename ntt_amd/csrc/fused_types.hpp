@@ -50,7 +50,10 @@ struct LimbConst {
     uint32_t q;
     uint32_t q_inv;
     uint32_t scale;    // Montgomery multiplier: forward input / inverse output
-    uint32_t flags;    // bit 0: apply scale (forward; the inverse always applies it)
+    // bit 0: apply scale (forward; the inverse always applies it).  With bit 0
+    // clear, scale must be R mod q (mont_mul by it is the identity): the lazy
+    // forward kernels fold scale into their first stage and apply it always.
+    uint32_t flags;
 };
 
 // Operands of `period` limbs of size 2^n stored back to back at `data`.  The

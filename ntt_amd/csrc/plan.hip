@@ -120,7 +120,9 @@ int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, siz
     P->d_lc_inv = P->d_lc_fwd + limbs;
     std::vector<LimbConst> lcf(limbs), lci(limbs);
     for (size_t l = 0; l < limbs; ++l) {
-        lcf[l] = {q[l], qi[l], sf[l], domain == NTT_DOMAIN_MONTGOMERY ? 1u : 0u};
+        // plain: no input scale, so the fused kernels' multiplier is R mod q (the identity)
+        lcf[l] = domain == NTT_DOMAIN_MONTGOMERY ? LimbConst{q[l], qi[l], sf[l], 1u}
+                                                 : LimbConst{q[l], qi[l], to_mont_host(1u, q[l]), 0u};
         lci[l] = {q[l], qi[l], si[l], 1u};
     }
     std::vector<char> h(2 * lcb + 4 * u32b + 2 * ptrb);
