@@ -28,11 +28,23 @@ static KernelFn pick_r(bool inv, bool col) {
     return phase_kernel<S, true, false, MODE, RF>;
 }
 
+// Column passes with the whole geometry fixed at compile time (logN 17,
+// R1 = 8, W = 5 or 6: the column splits the tuner picks for 2^17-word limbs,
+// which run on the two-pass kernels only).
 template <int S, int MODE>
-static KernelFn pick_m(bool inv, bool col, int R) {
+static KernelFn pick_col17(bool inv, int W) {
+    if (W == 5) return inv ? phase_kernel<S, true, true, MODE, 8, 17, 5> : phase_kernel<S, false, true, MODE, 8, 17, 5>;
+    return inv ? phase_kernel<S, true, true, MODE, 8, 17, 6> : phase_kernel<S, false, true, MODE, 8, 17, 6>;
+}
+
+template <int S, int MODE>
+static KernelFn pick_m(bool inv, bool col, int n, int R, int W) {
     // Compile-time radix for the row-pass splits the tuner picks at logN
-    // 16-17 (measured 8-15 % faster there; the column passes measured no
-    // better or slower, so they keep the runtime radix).
+    // 16-17 (measured 8-15 % faster there).  A column pass with only its
+    // radix fixed measured no better; with its whole geometry fixed
+    // (pick_col17) its LDS and global offsets become constants.
+    if (MODE == MODE_LAZY && S >= 4 && col && n == 17 && R == 8 && (W == 5 || W == 6))
+        return pick_col17<S, MODE>(inv, W);
     if (MODE != MODE_TWCONV && S >= 4 && !col) {
         if (R == 8) return pick_r<S, MODE, 8>(inv, col);
         if (R == 9) return pick_r<S, MODE, 9>(inv, col);
@@ -42,21 +54,21 @@ static KernelFn pick_m(bool inv, bool col, int R) {
 }
 
 template <int S>
-static KernelFn pick_s(bool inv, bool col, int mode, int R) {
+static KernelFn pick_s(bool inv, bool col, int mode, int n, int R, int W) {
     switch (mode) {
-        case MODE_CANON: return pick_m<S, MODE_CANON>(inv, col, R);
-        case MODE_TWCONV: return pick_m<S, MODE_TWCONV>(inv, col, R);
-        default: return pick_m<S, MODE_LAZY>(inv, col, R);
+        case MODE_CANON: return pick_m<S, MODE_CANON>(inv, col, n, R, W);
+        case MODE_TWCONV: return pick_m<S, MODE_TWCONV>(inv, col, n, R, W);
+        default: return pick_m<S, MODE_LAZY>(inv, col, n, R, W);
     }
 }
 
-static KernelFn pick(int S, bool inv, bool col, int mode, int R) {
+static KernelFn pick(int S, bool inv, bool col, int mode, int n, int R, int W) {
     switch (S) {
-        case 1: return pick_s<1>(inv, col, mode, R);
-        case 2: return pick_s<2>(inv, col, mode, R);
-        case 3: return pick_s<3>(inv, col, mode, R);
-        case 4: return pick_s<4>(inv, col, mode, R);
-        case 5: return pick_s<5>(inv, col, mode, R);
+        case 1: return pick_s<1>(inv, col, mode, n, R, W);
+        case 2: return pick_s<2>(inv, col, mode, n, R, W);
+        case 3: return pick_s<3>(inv, col, mode, n, R, W);
+        case 4: return pick_s<4>(inv, col, mode, n, R, W);
+        case 5: return pick_s<5>(inv, col, mode, n, R, W);
         default: return nullptr;
     }
 }
@@ -95,7 +107,7 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, int mode,
     p.xcd_remap = (base.xcd_remap && blocks % 8 == 0) ? 1 : 0;
     const unsigned threads = 1u << (R - S + W);
     const size_t lds = lds_bytes_for((size_t)1 << (R + W));
-    KernelFn fn = pick(S, inv, col, mode, R);
+    KernelFn fn = pick(S, inv, col, mode, p.logN, R, W);
     if (!fn) return NTT_ERR_CONFIG;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(fn),

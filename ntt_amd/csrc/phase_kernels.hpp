@@ -124,13 +124,18 @@ __device__ __forceinline__ void round_stages(uint32_t (&v)[1 << S], gcu32* __res
 // One pass.  COL: column pass (outer stages; forward phase 1 / inverse phase 2).
 // !COL: row pass (inner stages; forward phase 2 / inverse phase 1).
 // RF > 0: the pass's radix stages fixed at compile time (RF == p.R), so the
-// round windows and every index below fold to constants and shifts.
-template <int S, bool INV, bool COL, int MODE, int RF = 0>
+// round windows and every index below fold to constants and shifts.  NF, WF
+// > 0 (with RF): the transform size and the warp batching fixed as well
+// (NF == p.logN, WF == p.W), so the tile geometry, the column stride and
+// every LDS offset are constants too.
+template <int S, bool INV, bool COL, int MODE, int RF = 0, int NF = 0, int WF = 0>
 __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
     constexpr int E = 1 << S;
     extern __shared__ uint32_t lds[];
+    static_assert(!(NF || WF) || (NF && WF && RF), "NF and WF are set together, with RF");
 
-    const int n = p.logN, R = RF ? RF : p.R, W = p.W;
+    const int n = NF ? NF : p.logN, R = RF ? RF : p.R, W = WF ? WF : p.W;
+    const uint32_t tiles = NF ? (1u << (NF - RF - WF)) : p.tiles;
     const uint32_t t = threadIdx.x;
 
     // ---- workgroup -> (limb index, operand, tile) ------------------------
@@ -139,8 +144,8 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
         const uint32_t B = gridDim.x;   // host guarantees B % 8 == 0
         b = (b & 7u) * (B >> 3) + (b >> 3);
     }
-    const uint32_t tile = b % p.tiles;
-    const uint32_t rest = b / p.tiles;
+    const uint32_t tile = b % tiles;
+    const uint32_t rest = b / tiles;
     const uint32_t op = rest % p.count;
     const uint32_t lv = rest / p.count;
     const uint32_t li = lv + (lv >= p.skip_start ? p.skip_len : 0);   // plan limb
