@@ -102,8 +102,10 @@ static int launch_pass(const ntt_phase_config& pc, bool inv, bool col, int mode,
     p.R = R;
     p.W = W;
     p.tiles = 1u << (p.logN - R - W);
+    p.tiles_log2 = (uint32_t)(p.logN - R - W);
     const size_t blocks = (size_t)p.tiles * (p.limbs - p.skip_len) * p.count;
     if (blocks == 0 || blocks > 0x7fffffffu) return NTT_ERR_INVALID_ARGUMENT;
+    div_magic(p.count, &p.count_magic, &p.count_s1, &p.count_s2);
     p.xcd_remap = (base.xcd_remap && blocks % 8 == 0) ? 1 : 0;
     const unsigned threads = 1u << (R - S + W);
     const size_t lds = lds_bytes_for((size_t)1 << (R + W));

@@ -57,7 +57,11 @@ struct PhaseParams {
     int logN;                 // n
     int R;                    // stages in this pass (R1 for the column pass, R2 for the row pass)
     int W;                    // warp batching: log2 columns per tile / rows per workgroup
-    uint32_t tiles;           // workgroups per limb
+    uint32_t tiles;           // workgroups per limb (a power of two)
+    uint32_t tiles_log2;
+    // Division by count as a multiply-high: q = (t + ((x - t) >> s1)) >> s2
+    // with t = mulhi(x, count_magic) (launch_pass sets the three fields).
+    uint32_t count_magic, count_s1, count_s2;
     uint32_t limbs;           // limbs per operand
     uint32_t count;           // operands
     int xcd_remap;            // 1: group workgroups of the same limb index on one XCD
@@ -73,6 +77,26 @@ struct PhaseParams {
     uint32_t sub_start, sub_end;
     const uint32_t* subf;
 };
+
+// x / d for any 32-bit x, with (magic, s1, s2) from div_magic(d): the
+// round-up method for division by an invariant integer (Granlund and
+// Montgomery, "Division by invariant integers using multiplication", 1994).
+// With wave-uniform operands it runs on the scalar unit (s_mul_hi_u32)
+// instead of a VALU reciprocal sequence.
+__host__ __device__ __forceinline__ uint32_t div_by_magic(uint32_t x, uint32_t magic, uint32_t s1, uint32_t s2) {
+    const uint32_t t = (uint32_t)(((uint64_t)x * magic) >> 32);
+    return (t + ((x - t) >> s1)) >> s2;
+}
+
+// (magic, s1, s2) for divisor d >= 1: l = ceil(log2 d), magic = floor(2^32
+// (2^l - d) / d) + 1, s1 = min(l, 1), s2 = max(l - 1, 0).
+inline void div_magic(uint32_t d, uint32_t* magic, uint32_t* s1, uint32_t* s2) {
+    uint32_t l = 0;
+    while (l < 32 && ((uint64_t)1 << l) < d) ++l;
+    *magic = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+    *s1 = l < 1 ? l : 1;
+    *s2 = l > 1 ? l - 1 : 0;
+}
 
 // Arithmetic variants of the pass kernels.
 enum PassMode : int {
@@ -144,10 +168,10 @@ __global__ __launch_bounds__(1024) void phase_kernel(PhaseParams p) {
         const uint32_t B = gridDim.x;   // host guarantees B % 8 == 0
         b = (b & 7u) * (B >> 3) + (b >> 3);
     }
-    const uint32_t tile = b % tiles;
-    const uint32_t rest = b / tiles;
-    const uint32_t op = rest % p.count;
-    const uint32_t lv = rest / p.count;
+    const uint32_t tile = b & (tiles - 1);
+    const uint32_t rest = b >> (NF ? (NF - RF - WF) : p.tiles_log2);
+    const uint32_t lv = div_by_magic(rest, p.count_magic, p.count_s1, p.count_s2);
+    const uint32_t op = rest - lv * p.count;
     const uint32_t li = lv + (lv >= p.skip_start ? p.skip_len : 0);   // plan limb
     const size_t N = (size_t)1 << n;
     const size_t off = ((size_t)op * p.limbs + li) * N;
