@@ -277,16 +277,30 @@ def _timed(dev, stream, fn, reps):
     return e0.elapsed_ms(e1) / reps
 
 
+def phase_kernel_name(logN, phase, inv, col, mode):
+    """Symbol of the two-pass kernel launch_pass picks for one pass (R, S, W):
+    the radix fixed at compile time for row passes with R 8-10 and S >= 4
+    (not the table-converting mode 1), the whole geometry fixed for lazy
+    logN-17 column passes with R = 8 or 9, W = 5 or 6 (at most 1024 threads)."""
+    r, sm, w = phase
+    if mode == 2 and sm >= 4 and col and logN == 17 and r in (8, 9) and w in (5, 6) and r - sm + w <= 10:
+        rf, nf, wf = r, 17, w
+    elif mode != 1 and sm >= 4 and not col and r in (8, 9, 10):
+        rf, nf, wf = r, 0, 0
+    else:
+        rf, nf, wf = 0, 0, 0
+    b = lambda v: "true" if v else "false"
+    return f"phase_kernel<{sm}, {b(inv)}, {b(col)}, {mode}, {rf}, {nf}, {wf}>"
+
+
 def _pass_kernel_ms(dev, stream, plan, pool, count, cfg, lazy, twconv=False, steps=5):
     """Per-pass kernel times of a two-pass launch (HIP events on the launch
     stream), keyed by kernel symbol."""
     mode = 1 if twconv else (2 if lazy else 0)
     c = cfg.astuple()
 
-    def pk(ph, inv, col):   # the radix is a template argument for row passes, R 8-10, S >= 4
-        r, sm = c[ph][0], c[ph][1]
-        rf = r if (sm >= 4 and r in (8, 9, 10) and col == "false" and not twconv) else 0
-        return f"phase_kernel<{sm}, {inv}, {col}, {mode}, {rf}>"
+    def pk(ph, inv, col):
+        return phase_kernel_name(plan.logN, c[ph], inv == "true", col == "true", mode)
 
     kern = {pk(0, "false", "true"): lambda: plan.run_pass(pool, count, 0, 0, stream),
             pk(1, "false", "false"): lambda: plan.run_pass(pool, count, 0, 1, stream),
@@ -425,8 +439,10 @@ def bench_c1_plain(ntt_amd, dev, stream, traffic_path: str, logN: int = 16, limb
            "gbfly_s": round(2 * bfly / (step * 1e-3) / 1e9, 1),
            "hbm_frac": round(2 * 8 * words * count / (step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "phase_config": cfg,
-           "kernels": {"forward": f"phase_kernel<{cfg[0][1]}, false, true, 1, 0> + phase_kernel<{cfg[1][1]}, false, false, 1, 0>",
-                       "inverse": f"phase_kernel<{cfg[2][1]}, true, false, 1, 0> + phase_kernel<{cfg[3][1]}, true, true, 1, 0>"},
+           "kernels": {"forward": phase_kernel_name(logN, cfg[0], False, True, 1) + " + " +
+                                  phase_kernel_name(logN, cfg[1], False, False, 1),
+                       "inverse": phase_kernel_name(logN, cfg[2], True, False, 1) + " + " +
+                                  phase_kernel_name(logN, cfg[3], True, True, 1)},
            "roofline": {"bound": "hbm", "achieved": round(8 * words * count / (max(fwd, inv) * 1e-3) / 1e9, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": round(8 * words * count / (max(fwd, inv) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
@@ -670,10 +686,8 @@ def main():
     else:
         mode = 2 if lazy else 0
         c = cfg.astuple()
-        def pk(ph, inv, col):   # kernel symbol: the radix is a template argument for row passes, R 8-10, S >= 4
-            r, sm = c[ph][0], c[ph][1]
-            rf = r if (sm >= 4 and r in (8, 9, 10) and col == "false") else 0
-            return f"phase_kernel<{sm}, {inv}, {col}, {mode}, {rf}>"
+        def pk(ph, inv, col):
+            return phase_kernel_name(logN, c[ph], inv == "true", col == "true", mode)
         kern = {pk(0, "false", "true"): lambda: plan.run_pass(pool, B, 0, 0, stream),
                 pk(1, "false", "false"): lambda: plan.run_pass(pool, B, 0, 1, stream),
                 pk(2, "true", "false"): lambda: plan.run_pass(pool, B, 1, 0, stream),

@@ -28,13 +28,15 @@ static KernelFn pick_r(bool inv, bool col) {
     return phase_kernel<S, true, false, MODE, RF>;
 }
 
-// Column passes with the whole geometry fixed at compile time (logN 17,
-// R1 = 8, W = 5 or 6: the column splits the tuner picks for 2^17-word limbs,
-// which run on the two-pass kernels only).
-template <int S, int MODE>
+// Column passes with the whole geometry fixed at compile time: logN 17
+// (2^17-word limbs run on the two-pass kernels only), R1 = 8 or 9, W = 5 or
+// 6 within 1024 threads per workgroup.
+template <int S, int MODE, int R>
 static KernelFn pick_col17(bool inv, int W) {
-    if (W == 5) return inv ? phase_kernel<S, true, true, MODE, 8, 17, 5> : phase_kernel<S, false, true, MODE, 8, 17, 5>;
-    return inv ? phase_kernel<S, true, true, MODE, 8, 17, 6> : phase_kernel<S, false, true, MODE, 8, 17, 6>;
+    if (W == 5) return inv ? phase_kernel<S, true, true, MODE, R, 17, 5> : phase_kernel<S, false, true, MODE, R, 17, 5>;
+    if constexpr (R - S + 6 <= 10)
+        return inv ? phase_kernel<S, true, true, MODE, R, 17, 6> : phase_kernel<S, false, true, MODE, R, 17, 6>;
+    return nullptr;
 }
 
 template <int S, int MODE>
@@ -43,8 +45,10 @@ static KernelFn pick_m(bool inv, bool col, int n, int R, int W) {
     // 16-17 (measured 8-15 % faster there).  A column pass with only its
     // radix fixed measured no better; with its whole geometry fixed
     // (pick_col17) its LDS and global offsets become constants.
-    if (MODE == MODE_LAZY && S >= 4 && col && n == 17 && R == 8 && (W == 5 || W == 6))
-        return pick_col17<S, MODE>(inv, W);
+    if (MODE == MODE_LAZY && S >= 4 && col && n == 17 && (W == 5 || W == 6) && R - S + W <= 10) {
+        if (R == 8) return pick_col17<S, MODE, 8>(inv, W);
+        if (R == 9) return pick_col17<S, MODE, 9>(inv, W);
+    }
     if (MODE != MODE_TWCONV && S >= 4 && !col) {
         if (R == 8) return pick_r<S, MODE, 8>(inv, col);
         if (R == 9) return pick_r<S, MODE, 9>(inv, col);

@@ -178,17 +178,18 @@ def test_plans_match_oracle(env, logN, limbs, domain):
     assert ctx.twiddle_entries == 2 * limbs
 
 
-@pytest.mark.parametrize("col", [(8, 4, 5), (8, 4, 6), (8, 5, 5), (8, 5, 6)])
+@pytest.mark.parametrize("col", [(8, 4, 5), (8, 4, 6), (8, 5, 5), (8, 5, 6), (9, 4, 5), (9, 5, 5), (9, 5, 6)])
 @pytest.mark.parametrize("domain", [1, 0])
 def test_logN17_column_shapes(env, col, domain):
-    """The column passes compiled for one geometry (logN 17, R1 = 8, W = 5, 6;
-    picked whenever a two-pass launch at logN 17 uses that split), forward
+    """The column passes compiled for one geometry (logN 17, R1 = 8, 9, W = 5,
+    6; picked whenever a two-pass launch at logN 17 uses that split), forward
     and inverse, over several operands of several limbs."""
     ntt_amd, gu, orc = env
     logN, limbs, count = 17, 3, 2
     ctx = ntt_amd.Context(0)
     moduli = [tuple(m) for m in ntt_amd.generate_moduli(logN, limbs)]
-    cfg = ntt_amd.TransformConfig.make(col, (9, 4, 1), (9, 4, 1), col)
+    row = (17 - col[0], 4, 1)
+    cfg = ntt_amd.TransformConfig.make(col, row, row, col)
     plan = ctx.plan(logN, moduli, domain, cfg)
     plan.set_engine(ntt_amd.ENGINE_TWO_PASS)
     xs = [gu.random_operand(logN, moduli, 170 + 7 * k + domain) for k in range(count)]
