@@ -52,7 +52,9 @@ typedef enum { NTT_FORWARD = 0, NTT_INVERSE = 1 } ntt_direction;
 /* Kernel family a plan launches on.  TWO_PASS: the reference's two-phase
  * structure (column pass + row pass, one HBM round trip each).  FUSED: one
  * persistent workgroup per CU holds 2^16 words (one or more whole limbs) in
- * registers and LDS, one HBM round trip per transform (10 <= logN <= 16).
+ * registers and LDS, one HBM round trip per transform (10 <= logN <= 16);
+ * at logN 17 a workgroup transforms one limb as two 2^16-word halves in
+ * turn, 1.5 round trips (the intermediate half goes through the output).
  * AUTO: the autotuned choice for the launch size, else FUSED where supported. */
 typedef enum { NTT_ENGINE_AUTO = 0, NTT_ENGINE_TWO_PASS = 1, NTT_ENGINE_FUSED = 2 } ntt_engine;
 
@@ -104,7 +106,8 @@ const char* ntt_status_string(int status);
 /* Text of the last HIP error this thread's calls turned into NTT_ERR_HIP. */
 const char* ntt_last_hip_error(void);
 const char* ntt_version(void);
-/* Max logN supported (20).  The fused engine covers 10..16; the two-pass
+/* Max logN supported (20).  The fused engine covers 10..17 (17: one
+ * workgroup per limb, transformed as two 2^16-word halves in turn); the two-pass
  * engine every size, with the reference's phase split up to logN 18 and the
  * wave64 default split (ntt_default_transform_config) up to 20. */
 int ntt_max_log_n(void);
@@ -213,7 +216,8 @@ int ntt_plan_set_engine(ntt_plan* plan, int engine);
 int ntt_plan_set_constants(ntt_plan* plan, const uint32_t* prologue, const uint32_t* epilogue);
 /* Workgroup shape of the fused engine for both directions, per CU: 7 = 512
  * threads x 128 words each, 6 = 1024 threads x 64 words, 0 = the tuned
- * choice (default 6). */
+ * choice (default 6).  logN 17 always runs width 6 with table twiddles and
+ * has no fused mod-down form (ntt_plan_forward_mod_down runs two-pass there). */
 int ntt_plan_set_fused_shape(ntt_plan* plan, int eb);
 /* Words per fused workgroup, log2: 16 (one workgroup per CU, every fused
  * size), 15 (two independent workgroups per CU, logN <= 15; larger sizes
@@ -372,7 +376,7 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path);
  * set and direction, launched together.  No counterpart in the reference,
  * whose main() runs one fixed-size transform at a time
  * (src/v04_montgomery.cu:969-1062).  Jobs of one (logN, direction) with
- * 10 <= logN <= 16 run as ONE persistent fused kernel that walks a
+ * 10 <= logN <= 17 run as ONE persistent fused kernel that walks a
  * device-resident job table; jobs of other sizes run, per (logN, direction,
  * domain), as ONE two-pass transform over all their limbs through
  * device-resident per-limb operand pointers.

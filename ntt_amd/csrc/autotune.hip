@@ -211,13 +211,15 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
         // logN <= 15 both workgroup sizes (2^16 words, one per CU; 2^15
         // words, two per CU), and table or on-the-fly twiddles (the latter
         // at width 6, 2^16 words)
-        if (rc == NTT_OK && fused_supported(logN)) {
+        if (rc == NTT_OK && fused_plan_supported(logN)) {
             double bf = 0, bi = 0;
             for (int logw = 16; rc == NTT_OK && logw >= (logN <= 15 ? 15 : 16); --logw)
                 for (int eb = 7; rc == NTT_OK && eb >= 6; --eb)
                     for (int ot = 0; rc == NTT_OK && ot <= (eb == 6 && logw == 16 ? 1 : 0); ++ot)
                     for (int lz = P.lazy ? 1 : 0; rc == NTT_OK && lz >= 0; --lz)
                         for (int inv = 0; rc == NTT_OK && inv < 2; ++inv) {
+                            // split limbs (2^17 words): width 6, table twiddles only
+                            if (logN == FUSED_SPLIT_LOG_N && (eb != 6 || ot)) continue;
                             FusedVariant v;
                             v.eb = eb;
                             v.lazy = lz != 0;
@@ -333,7 +335,7 @@ int ntt_context_set_tuning(ntt_context* ctx, int logN, size_t total_limbs, const
     if (!ctx || !cfg || total_limbs == 0) return NTT_ERR_INVALID_ARGUMENT;
     int rc = validate_config(logN, cfg, NTT_LDS_BYTES_GFX950);
     if (rc) return rc;
-    if (engine == NTT_ENGINE_FUSED && !fused_supported(logN)) return NTT_ERR_CONFIG;
+    if (engine == NTT_ENGINE_FUSED && !fused_plan_supported(logN)) return NTT_ERR_CONFIG;
     TunedEntry e;
     e.cfg = *cfg;
     e.chunk = chunk;
@@ -396,7 +398,7 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         for (int i = 0; i < 4; ++i) p[i] = {v[3 * i], v[3 * i + 1], v[3 * i + 2]};
         if (validate_config(logN, &e.cfg, NTT_LDS_BYTES_GFX950) != NTT_OK) { rc = NTT_ERR_CONFIG; break; }
         e.chunk = chunk;
-        e.engine = (eng == NTT_ENGINE_FUSED && fused_supported(logN)) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
+        e.engine = (eng == NTT_ENGINE_FUSED && fused_plan_supported(logN)) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
         e.fused_eb_fwd = febf == 6 ? 6 : 7;
         e.fused_eb_inv = febi == 6 ? 6 : 7;
         e.fused_lazy = lz != 0;

@@ -222,7 +222,7 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         }
         b->job_plan[i] = it->second.get();
         b->jobs[i].moduli = nullptr;
-        if (fused_supported(j.logN))
+        if (fused_plan_supported(j.logN))
             by_shape[std::make_tuple(j.logN, j.direction == NTT_INVERSE ? 1 : 0, it->second->lazy ? 1 : 0)].push_back(i);
         else
             tp_shape[std::make_tuple(j.logN, j.direction == NTT_INVERSE ? 1 : 0, j.domain, it->second->lazy ? 1 : 0)]

@@ -122,5 +122,8 @@ void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, FusedVari
 FusedVariant fused_variant_for(const ntt_context* ctx, int logN, size_t total_limbs, bool inverse,
                                bool lazy_capable);
 bool fused_supported(int logN);
+// Sizes a plan can run on the fused engine: fused_supported, and 2^17-word
+// limbs as two halves (the split kernels).
+bool fused_plan_supported(int logN);
 int device_cus();
 }  // namespace ntt

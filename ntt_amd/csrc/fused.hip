@@ -26,8 +26,10 @@ int device_cus() {
 }
 
 bool fused_supported(int logN) { return logN >= FUSED_MIN_LOG_N && logN <= FUSED_MAX_LOG_N; }
+bool fused_plan_supported(int logN) { return fused_supported(logN) || logN == FUSED_SPLIT_LOG_N; }
 
 uint32_t fused_items(int logN, size_t limbs, int logw) {
+    if (logN == FUSED_SPLIT_LOG_N) return (uint32_t)limbs;   // one limb per workgroup
     const size_t lpw = (size_t)1 << (logw - logN);
     return (uint32_t)((limbs + lpw - 1) / lpw);
 }
@@ -38,6 +40,7 @@ FusedFn fused_fn_eb7(int n, bool inv, bool lazy);   // fused_eb7.hip
 FusedFn fused_fn_md(int n, bool lazy);               // fused_md.hip
 FusedFn fused_fn_ot(int n, bool inv, bool lazy);     // fused_ot.hip
 FusedFn fused_fn_w15(int n, bool inv, int eb, bool lazy);   // fused_w15.hip
+FusedFn fused_fn_split(bool inv, bool lazy);                 // fused_split.hip
 
 // Window width of the on-the-fly twiddle kernels.
 constexpr int OT_EB = 6;
@@ -79,6 +82,9 @@ int fused_prepare() {
                     attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds15);
             }
+        for (int d = 0; d < 4 && attr_err == hipSuccess; ++d)
+            attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn_split(d & 1, d >= 2)),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
             for (int lz = 0; lz < 2 && attr_err == hipSuccess; ++lz) {
                 FusedFn f = fused_fn_md(m, lz == 1);
@@ -113,6 +119,10 @@ int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipSt
                  unsigned grid_limit, int logw, bool ot) {
     if (p.total_items == 0) return NTT_OK;
     if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
+    if (n == FUSED_SPLIT_LOG_N) {
+        // split limbs: width 6, table twiddles, one limb per workgroup
+        return launch_fused_fn(fused_fn_split(inv, lazy), 6, p, stream, grid_limit);
+    }
     if (ot) {
         // on-the-fly twiddles: tables with the msb appendix, 1024 x 64 words per CU
         FusedFn f = fused_fn_ot(n, inv, lazy);
@@ -130,7 +140,7 @@ int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipSt
 int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
                       const FusedVariant& v, int stagger, const uint32_t* src, const uint32_t* sub,
                       const LimbMap& map) {
-    if (!fused_supported(P->logN)) return NTT_ERR_CONFIG;
+    if (!fused_plan_supported(P->logN)) return NTT_ERR_CONFIG;
     if ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(sub)) & 15)
         return NTT_ERR_INVALID_ARGUMENT;
     if (sub && (inverse || !P->d_post)) return NTT_ERR_INVALID_ARGUMENT;
@@ -163,6 +173,12 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.total_items = fused_items(P->logN, limbs, w15 ? 15 : 16);
     p.counter = nullptr;
     p.stagger = stagger < 0 ? 0u : (uint32_t)stagger;
+    if (P->logN == FUSED_SPLIT_LOG_N) {
+        // split limbs: table twiddles, width 6; no mod-down form
+        if (sub) return NTT_ERR_CONFIG;
+        if (p.total_items == 0) return NTT_OK;
+        return launch_fused(P->logN, inverse, 6, v.lazy && P->lazy, p, stream, 0, 16, false);
+    }
     if (v.ot && !sub) {
         // on-the-fly twiddles: 1024 x 64 words, lazy when every q < 2^30 (and
         // the variant asks for it)

@@ -154,9 +154,15 @@ inline constexpr uint32_t LDS_OFF = lds_pad<EB>(compress<S>((uint32_t)X << LO));
 // TL: the stage twiddles (and the on-the-fly factors) come from the staged
 // LDS copy twl (tw[0 .. FUSED_TWL_TW - 1], then the msb appendix) instead of
 // the table in HBM.
-template <int n, bool INV, bool LAZY, int K, int EB, bool OT = false, bool TL = false>
+// SPLIT: the transform is one half of a 2^(n+1)-word limb (after the forward's
+// cross-half stage, before the inverse's), whose stage twiddles are those of
+// the 2^(n+1)-word table at (tb0 m + g) for level m and group g, with
+// tb0 = 2 + half (tb0 = 1: a whole limb); the lazy forward's input scale is
+// then applied at the cross-half stage, not here.
+template <int n, bool INV, bool LAZY, int K, int EB, bool OT = false, bool TL = false, bool SPLIT = false>
 __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict__ tw, const uint32_t* twl,
-                                       uint32_t u, uint32_t q, uint32_t q_inv, uint32_t scale = 0) {
+                                       uint32_t u, uint32_t q, uint32_t q_inv, uint32_t scale = 0,
+                                       uint32_t tb0 = 1) {
     constexpr int E = 1 << EB;
     constexpr int LO = win_lo(n, INV, K, EB);
     constexpr int JLO = stage_jlo(n, INV, K, EB), JHI = stage_jhi(n, INV, K, EB);
@@ -166,7 +172,7 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
     // (wave-uniform) table base, read with scalar loads into SGPRs.  (The
     // inverse's last round has the same property, but there the SGPR
     // twiddles raised the lazy kernels' spills and measured slower.)
-    const uint32_t TB = (!INV && LO + EB == n) ? 1u : (1u << (n - EB - LO)) + (opaque(u) >> LO);
+    const uint32_t TB = (!INV && LO + EB == n) ? tb0 : (tb0 << (n - EB - LO)) + (opaque(u) >> LO);
     // The inverse's last stage (bit n - 1, twiddle tw[1] for every word) is
     // folded into the output scaling by run_item.
     constexpr bool FOLDED_TOP = INV && K == nrounds(n, EB) - 1;
@@ -177,7 +183,7 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
     // pair) applies the input scale s as well: x' = x s / R and
     // y' = y (s w) / R, both lazy in [0, 2q), then (x' + y', x' + 2q - y')
     // in [0, 4q): two Montgomery products per pair instead of three.
-    constexpr bool FOLD_SCALE = LAZY && !INV && K == 0;
+    constexpr bool FOLD_SCALE = LAZY && !INV && K == 0 && !SPLIT;
     static_for<EB>([&](auto JJ) __attribute__((always_inline)) {
         constexpr int j = INV ? decltype(JJ)::value : EB - 1 - decltype(JJ)::value;
         if constexpr (FOLD_SCALE && j == EB - 1) {
@@ -315,12 +321,83 @@ __device__ __forceinline__ FusedJob uniform_job(const FusedJob& j) {
     return u;
 }
 
+// Word pairs per chunk of the split kernels' pipelined loads (two chunks in
+// flight beside the 64 words a thread holds).
+constexpr int SPLIT_LOAD_CHUNK = 8;
+
+// The end of half `half` of a split inverse (run_item): the half's top
+// stage (bit n - 1, twiddle tw[2 + half]).  Half 0 leaves its result T0 in
+// half 0 of the output; half 1 reads T0 back at the positions this thread
+// stored it (so no other thread's stores are involved) and computes the
+// cross-half stage (bit n, twiddle tw[1]) folded with the output scale:
+// ((a + b) s, (a - b) (s w) / R) / R, a = T0, b = T1, for both halves.
+template <int n, bool LAZY, int EB>
+__device__ __forceinline__ void split_inverse_store(uint32_t (&v)[1 << EB], const FusedJob& job, size_t dlimb,
+                                                    uint32_t u, gcu32* tw, const LimbConst& c, uint32_t half) {
+    constexpr int E = 1 << EB;
+    constexpr int NR = nrounds(n, EB);
+    constexpr int LOL = win_lo(n, true, NR - 1, EB);
+    const uint32_t w = tw[2 + half];
+    static_for<E / 2>([&](auto P) __attribute__((always_inline)) {
+        constexpr int x = decltype(P)::value, y = x | (E / 2);
+        constexpr int ra = REG<n, true, NR - 1, x, EB>, rb = REG<n, true, NR - 1, y, EB>;
+        if (LAZY)
+            gs_butterfly_lazy(v[ra], v[rb], w, c.q, c.q_inv);
+        else
+            gs_butterfly(v[ra], v[rb], w, c.q, c.q_inv);
+    });
+    gu32* const O = (gu32*)(job.data + (dlimb << (n + 1))) + ubase<LOL, EB>(opaque(u));
+    if (half == 0) {
+        static_for<E>([&](auto X) __attribute__((always_inline)) {
+            constexpr int x = decltype(X)::value;
+            O[x << LOL] = v[REG<n, true, NR - 1, x, EB>];
+        });
+        return;
+    }
+    const uint32_t s2 = mont_mul(c.scale, tw[1], c.q, c.q_inv);
+    const uint32_t qq = LAZY ? 2 * c.q : c.q;
+    // software-pipelined: the loads of chunk k + 1 are issued before chunk k
+    // is combined and stored
+    constexpr int NCH = E / SPLIT_LOAD_CHUNK;
+    uint32_t to[2][SPLIT_LOAD_CHUNK];
+    static_for<NCH + 1>([&](auto K) __attribute__((always_inline)) {
+        constexpr int k = decltype(K)::value;
+        if constexpr (k < NCH) {
+            static_for<SPLIT_LOAD_CHUNK>([&](auto J) __attribute__((always_inline)) {
+                constexpr int x = k * SPLIT_LOAD_CHUNK + decltype(J)::value;
+                to[k & 1][decltype(J)::value] = O[x << LOL];
+            });
+        }
+        asm volatile("" ::: "memory");
+        if constexpr (k > 0) {
+            static_for<SPLIT_LOAD_CHUNK>([&](auto J) __attribute__((always_inline)) {
+                constexpr int j = decltype(J)::value, x = (k - 1) * SPLIT_LOAD_CHUNK + j;
+                const uint32_t a = to[(k - 1) & 1][j], b = v[REG<n, true, NR - 1, x, EB>];
+                O[x << LOL] = mont_mul(a + b, c.scale, c.q, c.q_inv);
+                O[(1u << n) + (x << LOL)] = mont_mul(a + qq - b, s2, c.q, c.q_inv);
+            });
+        }
+    });
+}
+
 // TWL: one limb of 2^16 words per workgroup with on-the-fly twiddles; the
 // twiddle words the rounds read are staged into LDS after the exchange
 // buffer whenever the limb's table differs from the staged one (*staged).
-template <int n, bool INV, bool LAZY, int EB, bool MD, bool OT, bool TWL = false>
+// SPLIT: the item is a limb of 2^(n+1) words, transformed as its two halves
+// one after the other by this workgroup (so no other workgroup reads or
+// writes the limb).  Forward: the first pass reads both halves, computes the
+// cross-half stage (bit n, twiddle tw[1]) with the input scale, keeps half 0
+// in registers and stores half 1's stage output in half 1 of the output (the
+// words each thread has just read there); then each half's 2^n-word
+// sub-transform runs with the twiddles of the 2^(n+1)-word table (stages),
+// half 1's reloading the words this thread stored.  Inverse: half 0's
+// sub-transform leaves its result in half 0 of the output, and half 1's
+// combines it with its own (split_inverse_store).
+template <int n, bool INV, bool LAZY, int EB, bool MD, bool OT, bool TWL = false, bool SPLIT = false>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds,
                                          const uint32_t** staged) {
+    static_assert(!SPLIT || (n == 16 && EB == 6 && !MD && !OT), "split limbs: 2 x 2^16 words, width 6, table twiddles");
+    constexpr int NL = SPLIT ? n + 1 : n;   // log2 of the limb stride
     constexpr int E = 1 << EB;
     constexpr int G_BITS = n - EB;
     constexpr uint32_t STRIDE = (1u << (n - 1)) + (1u << (n - 1 - pad_shift(EB)));
@@ -344,7 +421,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     const uint32_t op = lsafe / vper, pv = lsafe - op * vper;
     const uint32_t per = pv + (pv >= job.skip_start ? job.skip_len : 0);
     const size_t dlimb = (size_t)op * job.period + per;
-    gcu32* In = (gcu32*)(job.src + (((size_t)op * job.src_period + job.src_first + per) << n));
+    gcu32* const InL = (gcu32*)(job.src + (((size_t)op * job.src_period + job.src_first + per) << NL));
     gcu32* tw = (gcu32*)job.tw[per];
     LimbConst c = job.lc[per];
     if constexpr (G_BITS >= 6) {   // one limb per wave: the limb's table and constants are wave-uniform
@@ -368,9 +445,64 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         }
     }
 
+#pragma clang loop unroll(disable)
+    for (uint32_t half = 0; half < (SPLIT ? 2u : 1u); ++half) {
+    gcu32* const In = InL + (SPLIT && INV ? half << n : 0u);
     uint32_t v[E];
     // ---- load (round 0 layout) ----
-    if (valid) {
+    if (SPLIT && !INV && half == 1) {
+        // half 1's cross-half stage output, stored by this thread in pass 0
+        gcu32* Ab = (gcu32*)(job.data + (dlimb << NL)) + (1u << n) + ubase<LO0, EB>(opaque(u));
+        static_for<E>([&](auto X) __attribute__((always_inline)) {
+            constexpr int x = decltype(X)::value;
+            v[REG<n, INV, 0, x, EB>] = Ab[x << LO0];
+        });
+    } else if constexpr (SPLIT && !INV) {
+        // the cross-half stage (bit n, twiddle tw[1]) with the input scale s:
+        // x' = x s / R, y' = y (s w) / R, then x' + y' (half 0, kept) and
+        // x' - y' (half 1, stored); lazy: both in [0, 2q), results in [0, 4q)
+        const uint32_t w1 = tw[1];
+        const bool sc = LAZY || (c.flags & 1);
+        const uint32_t sw = mont_mul(sc ? c.scale : 1u, w1, c.q, c.q_inv);
+        const uint32_t yf = sc ? sw : w1;   // canonical without scale: y w
+        gcu32* A0 = In + ubase<LO0, EB>(opaque(u));
+        gcu32* A1 = A0 + (1u << n);
+        gu32* D1 = (gu32*)(job.data + (dlimb << NL)) + (1u << n) + ubase<LO0, EB>(opaque(u));
+        const uint32_t q2 = 2 * c.q;
+        // software-pipelined in chunks of SCH word pairs: the loads of chunk
+        // k + 1 are issued before chunk k is combined (two chunks of loads in
+        // flight beside the 64 words; all 128 loads at once would spill)
+        constexpr int NCH = E / SPLIT_LOAD_CHUNK;
+        uint32_t ta[2][SPLIT_LOAD_CHUNK], tb[2][SPLIT_LOAD_CHUNK];
+        static_for<NCH + 1>([&](auto K) __attribute__((always_inline)) {
+            constexpr int k = decltype(K)::value;
+            if constexpr (k < NCH) {
+                static_for<SPLIT_LOAD_CHUNK>([&](auto J) __attribute__((always_inline)) {
+                    constexpr int x = k * SPLIT_LOAD_CHUNK + decltype(J)::value;
+                    ta[k & 1][decltype(J)::value] = A0[x << LO0];
+                    tb[k & 1][decltype(J)::value] = A1[x << LO0];
+                });
+            }
+            asm volatile("" ::: "memory");
+            if constexpr (k > 0) {
+                static_for<SPLIT_LOAD_CHUNK>([&](auto J) __attribute__((always_inline)) {
+                    constexpr int j = decltype(J)::value, x = (k - 1) * SPLIT_LOAD_CHUNK + j;
+                    const uint32_t a = ta[(k - 1) & 1][j], b = tb[(k - 1) & 1][j];
+                    if constexpr (LAZY) {
+                        const uint32_t xs = mont_mul_lazy(opaque(a), c.scale, c.q, c.q_inv);
+                        const uint32_t ys = mont_mul_lazy(opaque(b), sw, c.q, c.q_inv);
+                        v[REG<n, INV, 0, x, EB>] = xs + ys;
+                        D1[x << LO0] = xs + q2 - ys;
+                    } else {
+                        const uint32_t xs = sc ? mont_mul(a, c.scale, c.q, c.q_inv) : a;
+                        const uint32_t ys = mont_mul(b, yf, c.q, c.q_inv);
+                        v[REG<n, INV, 0, x, EB>] = add_mod(xs, ys, c.q);
+                        D1[x << LO0] = sub_mod(xs, ys, c.q);
+                    }
+                });
+            }
+        });
+    } else if (valid) {
         if (LO0 == 0) {
             gcu4* src = (gcu4*)(In + ubase<LO0, EB>(opaque(u)));
             static_for<E / 4>([&](auto X4) __attribute__((always_inline)) {
@@ -389,7 +521,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
             });
         }
         // the input scale (the lazy forward applies it in its first stage)
-        if (!INV && !LAZY && (c.flags & 1)) {
+        if (!INV && !LAZY && !SPLIT && (c.flags & 1)) {
 #pragma unroll
             for (int x = 0; x < E; ++x) v[x] = mont_mul(v[x], c.scale, c.q, c.q_inv);
         }
@@ -400,12 +532,13 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 
     // staged twiddles: every round of the inverse, the forward's rounds after
     // the first (whose twiddles tw[1 .. 63] are scalar loads)
-    stages<n, INV, LAZY, 0, EB, OT, TWL && INV>(v, tw, twl, u, c.q, c.q_inv, c.scale);
+    const uint32_t tb0 = SPLIT ? 2u + half : 1u;
+    stages<n, INV, LAZY, 0, EB, OT, TWL && INV, SPLIT>(v, tw, twl, u, c.q, c.q_inv, c.scale, tb0);
     exchange<n, INV, 0, EB>(v, L, u, valid);
-    stages<n, INV, LAZY, 1, EB, OT, TWL>(v, tw, twl, u, c.q, c.q_inv);
+    stages<n, INV, LAZY, 1, EB, OT, TWL, SPLIT>(v, tw, twl, u, c.q, c.q_inv, 0, tb0);
     if constexpr (NR == 3) {
         exchange<n, INV, 1, EB>(v, L, u, valid);
-        stages<n, INV, LAZY, 2, EB, OT, TWL>(v, tw, twl, u, c.q, c.q_inv);
+        stages<n, INV, LAZY, 2, EB, OT, TWL, SPLIT>(v, tw, twl, u, c.q, c.q_inv, 0, tb0);
     }
 
     // ---- store (last round layout) ----
@@ -414,6 +547,10 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
     // schedule interleaves the round's products and spills.
 #pragma unroll
     for (int x = 0; x < E; ++x) asm volatile("" : "+v"(v[x]));
+    if constexpr (SPLIT && INV) {
+        split_inverse_store<n, LAZY, EB>(v, job, dlimb, u, tw, c, half);
+        continue;
+    }
     if (valid) {
         if (INV) {
             // the last stage (bit n - 1, twiddle w = itw[1]) folded with the
@@ -466,7 +603,7 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
                 for (int x = 0; x < E; ++x) v[x] = mont_mul(sub_mod(0u, v[x], c.q), f, c.q, c.q_inv);
             }
         }
-        gu32* A = (gu32*)(job.data + (dlimb << n));
+        gu32* A = (gu32*)(job.data + (dlimb << NL)) + (SPLIT ? half << n : 0u);
         if constexpr (XST) {
             // The wave's 64 threads hold 4096 consecutive words, lane l the 64
             // words (16 chunks of 16 B) at 64 l.  Stored as they are, each
@@ -521,8 +658,10 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
             });
         }
     }
-    // the next item's first exchange writes the LDS the transposes read
+    // the next item's (or half's) first exchange writes the LDS the
+    // transposes read
     if constexpr (XST) __syncthreads();
+    }   // halves
 }
 
 }  // namespace fused
@@ -534,16 +673,18 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
 // OT: on-the-fly twiddles in the low-bit round (stages).
 // The register budget is that of 2^(16-EB) threads per CU (2^(8-EB) waves
 // per SIMD) for either LOGW.
-template <int n, bool INV, bool LAZY, int EB, bool MD = false, bool OT = false, int LOGW = 16>
-__global__ __launch_bounds__(1 << (LOGW - EB)) __attribute__((amdgpu_waves_per_eu(1 << (8 - EB))))
-void fused_kernel(FusedParams p) {
+// SPLIT (n = 16): an item is one limb of 2^17 words (run_item).
+template <int n, bool INV, bool LAZY, int EB, bool MD, bool OT, int LOGW, bool SPLIT>
+__device__ __forceinline__ void fused_body(FusedParams p) {
     static_assert(n <= LOGW && LOGW >= 15 && LOGW <= 16, "a workgroup holds whole limbs of at most 2^LOGW words");
+    static_assert(!SPLIT || (n == 16 && LOGW == 16), "split limbs: two 2^16-word halves");
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ uint32_t s_next;
     uint32_t iter = 0;
     constexpr bool TWL = OT && n == 16 && LOGW == 16 && EB == 6;
     const uint32_t* staged = nullptr;   // table whose words the LDS holds (TWL)
     for (uint32_t s = 0, ns = (blockIdx.x & 3u) * p.stagger; s < ns; ++s) __builtin_amdgcn_s_sleep(127);
+    const uint32_t total = p.total_items;
     for (;;) {
         uint32_t item;
         if (p.counter) {
@@ -556,7 +697,7 @@ void fused_kernel(FusedParams p) {
             ++iter;
         }
         item = __builtin_amdgcn_readfirstlane(item);
-        if (item >= p.total_items) {
+        if (item >= total) {
             // dynamic scheduling: every workgroup leaves after one claim past
             // the end, so the last one out (counter[1] reaching the grid size)
             // is the last to touch the counters and returns both to zero for
@@ -578,8 +719,23 @@ void fused_kernel(FusedParams p) {
             job = p.jobs[lo];
         }
         job = fused::uniform_job(job);
-        fused::run_item<n, INV, LAZY, EB, MD, OT, TWL>(job, (item - job.item_begin) << (LOGW - n), lds, &staged);
+        fused::run_item<n, INV, LAZY, EB, MD, OT, TWL && !SPLIT, SPLIT>(job, (item - job.item_begin) << (LOGW - n), lds,
+                                                                      &staged);
     }
+}
+
+template <int n, bool INV, bool LAZY, int EB, bool MD = false, bool OT = false, int LOGW = 16>
+__global__ __launch_bounds__(1 << (LOGW - EB)) __attribute__((amdgpu_waves_per_eu(1 << (8 - EB))))
+void fused_kernel(FusedParams p) {
+    fused_body<n, INV, LAZY, EB, MD, OT, LOGW, false>(p);
+}
+
+// Limbs of 2^17 words as two 2^16-word halves (fused_body, SPLIT): width 6,
+// table twiddles, one workgroup of 1024 threads per CU.
+template <bool INV, bool LAZY>
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
+void fused_split_kernel(FusedParams p) {
+    fused_body<16, INV, LAZY, 6, false, false, 16, true>(p);
 }
 
 }  // namespace ntt

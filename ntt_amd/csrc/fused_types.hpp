@@ -111,5 +111,8 @@ struct FusedParams {
     // with the compute phases of others instead of all hitting HBM at once.
     uint32_t stagger;
 };
+// The split kernels: limbs of 2^17 words, one per workgroup, transformed as
+// two 2^16-word halves in turn.
+constexpr int FUSED_SPLIT_LOG_N = 17;
 
 }  // namespace ntt

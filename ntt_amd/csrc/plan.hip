@@ -170,7 +170,7 @@ int plan_engine(const ntt_plan* P, size_t count) {
     TunedEntry t;
     if (!P->cfg_fixed && P->ctx->tuned_lookup(P->logN, P->limbs * count, &t)) return t.engine;
     if (P->cfg_fixed) return NTT_ENGINE_TWO_PASS;
-    return fused_supported(P->logN) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
+    return fused_plan_supported(P->logN) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
 }
 
 // Tuned (or default) variant of one direction; 32K-word workgroups only
@@ -329,7 +329,9 @@ static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse,
     // sub is read in the last pass, after A already holds intermediate values
     const size_t sub_bytes = (plan->sub_range ? plan->sub_end - plan->sub_start : plan->limbs) * N * count * 4;
     if (sub && (sub == A_dev || overlap(sub, sub_bytes, A_dev, bytes))) return NTT_ERR_INVALID_ARGUMENT;
-    if (plan_engine(plan, count) == NTT_ENGINE_FUSED) {
+    // the split (2^17-word) fused kernels have no mod-down form: at that size
+    // it runs on the two-pass kernels
+    if (plan_engine(plan, count) == NTT_ENGINE_FUSED && !(sub && plan->logN == FUSED_SPLIT_LOG_N)) {
         int stagger = 0;
         FusedVariant v;
         plan_fused_variant(plan, count, inverse, &v, &stagger);
@@ -362,7 +364,7 @@ int ntt_plan_set_engine(ntt_plan* plan, int engine) {
     if (!plan) return NTT_ERR_INVALID_ARGUMENT;
     if (engine != NTT_ENGINE_AUTO && engine != NTT_ENGINE_TWO_PASS && engine != NTT_ENGINE_FUSED)
         return NTT_ERR_INVALID_ARGUMENT;
-    if (engine == NTT_ENGINE_FUSED && !fused_supported(plan->logN)) return NTT_ERR_CONFIG;
+    if (engine == NTT_ENGINE_FUSED && !fused_plan_supported(plan->logN)) return NTT_ERR_CONFIG;
     plan->engine = engine;
     return NTT_OK;
 }

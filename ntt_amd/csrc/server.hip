@@ -274,7 +274,7 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
         // (fused when the size is untuned)
         TunedEntry tuned;
         const bool tuned_two_pass = S->ctx->tuned_lookup(n, tot, &tuned) && tuned.engine == NTT_ENGINE_TWO_PASS;
-        if (fused_supported(n) && !tuned_two_pass) {
+        if (fused_plan_supported(n) && !tuned_two_pass) {
             const size_t first = nd;
             const FusedVariant v = fused_variant_for(S->ctx, n, tot, inv, std::get<2>(kv.first) != 0);
             uint64_t items = 0;
