@@ -262,6 +262,8 @@ def fused_kernel_name(logN: int, inverse: bool, eb: int, lazy: bool, logw: int, 
     """Symbol of the fused kernel a plan launch runs, with its template
     arguments as the kernel trace prints them."""
     tf = lambda b: "true" if b else "false"
+    if logN == 17:   # one limb per workgroup as two 2^16-word halves: width 6, OT inverse only
+        return f"fused_split_kernel<{tf(inverse)}, {tf(lazy)}, {tf(ot and inverse)}>"
     return f"fused_kernel<{logN}, {tf(inverse)}, {tf(lazy)}, {eb}, false, {tf(ot)}, {logw}>"
 
 
@@ -357,7 +359,14 @@ def bench_c3(ntt_amd, dev, stream, traffic_path: str, tune: bool = True, count: 
     ok = bool(np.array_equal(before, pool.numpy(np.uint32, nbytes=4 * words)))
     bfly = 0.5 * N * logN * limbs * count
     eng = {1: "two_pass", 2: "fused"}.get(plan.engine(count), "?")
-    kms = _pass_kernel_ms(dev, stream, plan, pool, count, cfg, all(q < (1 << 30) for q, _ in moduli))
+    if eng == "fused":
+        # one kernel per plan launch (the split kernels, one limb per workgroup)
+        ebf, lzf, lwf, otf = plan.fused_variant_full(count, ntt_amd.FORWARD)
+        ebi, lzi, lwi, oti = plan.fused_variant_full(count, ntt_amd.INVERSE)
+        kms = {fused_kernel_name(logN, False, ebf, lzf, lwf, otf): fwd,
+               fused_kernel_name(logN, True, ebi, lzi, lwi, oti): inv}
+    else:
+        kms = _pass_kernel_ms(dev, stream, plan, pool, count, cfg, all(q < (1 << 30) for q, _ in moduli))
     step = fwd + inv
     out = {"workload": f"N=2^17, 48 limbs (48 largest primes < 2^30, q = 1 mod 2^18), Montgomery, {count} operands "
                        f"({4 * words * count / 2**30:.2f} GiB), forward then inverse, seed 0",

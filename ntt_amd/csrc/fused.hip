@@ -40,7 +40,7 @@ FusedFn fused_fn_eb7(int n, bool inv, bool lazy);   // fused_eb7.hip
 FusedFn fused_fn_md(int n, bool lazy);               // fused_md.hip
 FusedFn fused_fn_ot(int n, bool inv, bool lazy);     // fused_ot.hip
 FusedFn fused_fn_w15(int n, bool inv, int eb, bool lazy);   // fused_w15.hip
-FusedFn fused_fn_split(bool inv, bool lazy);                 // fused_split.hip
+FusedFn fused_fn_split(bool inv, bool lazy, bool ot);        // fused_split.hip
 
 // Window width of the on-the-fly twiddle kernels.
 constexpr int OT_EB = 6;
@@ -82,9 +82,11 @@ int fused_prepare() {
                     attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(f),
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds15);
             }
-        for (int d = 0; d < 4 && attr_err == hipSuccess; ++d)
-            attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn_split(d & 1, d >= 2)),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        for (int d = 0; d < 8 && attr_err == hipSuccess; ++d)
+            if (!(d & 4) || (d & 1))   // on-the-fly twiddles: inverse only
+                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn_split(d & 1, d & 2, d & 4)),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)fused_split_lds_bytes(d & 4));
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
             for (int lz = 0; lz < 2 && attr_err == hipSuccess; ++lz) {
                 FusedFn f = fused_fn_md(m, lz == 1);
@@ -99,10 +101,10 @@ int fused_prepare() {
 // One launch of fused kernel `fn` (window width eb, 2^logw words per
 // workgroup, 2^(16-logw) workgroups per CU) over the items of p.
 static int launch_fused_fn(FusedFn fn, int eb, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
-                           int logw = 16) {
+                           int logw = 16, size_t lds_bytes = 0) {
     int rc = fused_prepare();
     if (rc) return rc;
-    const size_t lds = fused_lds_bytes(logw);
+    const size_t lds = lds_bytes ? lds_bytes : fused_lds_bytes(logw);
     unsigned grid = (unsigned)device_cus() << (16 - logw);
     if (grid_limit && grid_limit < grid) grid = grid_limit;
     if (p.total_items < grid) grid = p.total_items;
@@ -120,8 +122,11 @@ int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipSt
     if (p.total_items == 0) return NTT_OK;
     if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
     if (n == FUSED_SPLIT_LOG_N) {
-        // split limbs: width 6, table twiddles, one limb per workgroup
-        return launch_fused_fn(fused_fn_split(inv, lazy), 6, p, stream, grid_limit);
+        // split limbs: width 6, one limb per workgroup; on-the-fly twiddles
+        // for the inverse only
+        const bool sot = ot && inv;
+        return launch_fused_fn(fused_fn_split(inv, lazy, sot), 6, p, stream, grid_limit, 16,
+                               fused_split_lds_bytes(sot));
     }
     if (ot) {
         // on-the-fly twiddles: tables with the msb appendix, 1024 x 64 words per CU
@@ -177,7 +182,7 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
         // split limbs: table twiddles, width 6; no mod-down form
         if (sub) return NTT_ERR_CONFIG;
         if (p.total_items == 0) return NTT_OK;
-        return launch_fused(P->logN, inverse, 6, v.lazy && P->lazy, p, stream, 0, 16, false);
+        return launch_fused(P->logN, inverse, 6, v.lazy && P->lazy, p, stream, 0, 16, v.ot && inverse);
     }
     if (v.ot && !sub) {
         // on-the-fly twiddles: 1024 x 64 words, lazy when every q < 2^30 (and

@@ -209,7 +209,9 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
             constexpr bool OTS = OT && LO == 0 && k <= OT_LSB_BITS;
             uint32_t ot_msb = 0, ot_lo = 0;
             if constexpr (OTS) {
-                ot_msb = TL ? twl[FUSED_TWL_TW + (TB >> (OT_LSB_BITS - k))] : tw[(1u << n) + (TB >> (OT_LSB_BITS - k))];
+                // the msb appendix follows the table (of 2^(n+1) words for a split limb)
+                constexpr uint32_t MSB = 1u << (SPLIT ? n + 1 : n);
+                ot_msb = TL ? twl[FUSED_TWL_TW + (TB >> (OT_LSB_BITS - k))] : tw[MSB + (TB >> (OT_LSB_BITS - k))];
                 ot_lo = (TB & ((1u << (OT_LSB_BITS - k)) - 1)) << k;
             }
             static_for<E / 2>([&](auto P) __attribute__((always_inline)) {
@@ -396,7 +398,8 @@ __device__ __forceinline__ void split_inverse_store(uint32_t (&v)[1 << EB], cons
 template <int n, bool INV, bool LAZY, int EB, bool MD, bool OT, bool TWL = false, bool SPLIT = false>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds,
                                          const uint32_t** staged) {
-    static_assert(!SPLIT || (n == 16 && EB == 6 && !MD && !OT), "split limbs: 2 x 2^16 words, width 6, table twiddles");
+    static_assert(!SPLIT || (n == 16 && EB == 6 && !MD && (!OT || INV)),
+                  "split limbs: 2 x 2^16 words, width 6, on-the-fly twiddles for the inverse only");
     constexpr int NL = SPLIT ? n + 1 : n;   // log2 of the limb stride
     constexpr int E = 1 << EB;
     constexpr int G_BITS = n - EB;
@@ -437,9 +440,12 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         static_assert(n == 16 && OT, "staged twiddles: one 2^16-word limb per workgroup, on-the-fly twiddles");
         if ((const uint32_t*)tw != *staged) {   // uniform: the table base is in SGPRs
             __syncthreads();   // the previous item's rounds have read the staged words
+            // a split limb's table has 2^(n+1) words and an appendix of 2^(n-5)
+            // (its inverse's later rounds read tw[0 .. 2047] as well)
+            constexpr uint32_t MSB_WORDS = SPLIT ? FUSED_SPLIT_TWL_WORDS - FUSED_TWL_TW : FUSED_TWL_WORDS - FUSED_TWL_TW;
             for (uint32_t i = threadIdx.x; i < FUSED_TWL_TW; i += blockDim.x) twl[i] = tw[i];
-            for (uint32_t i = threadIdx.x; i < FUSED_TWL_WORDS - FUSED_TWL_TW; i += blockDim.x)
-                twl[FUSED_TWL_TW + i] = tw[(1u << n) + i];
+            for (uint32_t i = threadIdx.x; i < MSB_WORDS; i += blockDim.x)
+                twl[FUSED_TWL_TW + i] = tw[(1u << NL) + i];
             __syncthreads();
             *staged = (const uint32_t*)tw;
         }
@@ -719,7 +725,7 @@ __device__ __forceinline__ void fused_body(FusedParams p) {
             job = p.jobs[lo];
         }
         job = fused::uniform_job(job);
-        fused::run_item<n, INV, LAZY, EB, MD, OT, TWL && !SPLIT, SPLIT>(job, (item - job.item_begin) << (LOGW - n), lds,
+        fused::run_item<n, INV, LAZY, EB, MD, OT, TWL, SPLIT>(job, (item - job.item_begin) << (LOGW - n), lds,
                                                                       &staged);
     }
 }
@@ -732,10 +738,10 @@ void fused_kernel(FusedParams p) {
 
 // Limbs of 2^17 words as two 2^16-word halves (fused_body, SPLIT): width 6,
 // table twiddles, one workgroup of 1024 threads per CU.
-template <bool INV, bool LAZY>
+template <bool INV, bool LAZY, bool OT = false>
 __global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(4)))
 void fused_split_kernel(FusedParams p) {
-    fused_body<16, INV, LAZY, 6, false, false, 16, true>(p);
+    fused_body<16, INV, LAZY, 6, false, OT, 16, true>(p);
 }
 
 }  // namespace ntt

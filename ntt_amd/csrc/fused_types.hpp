@@ -114,5 +114,12 @@ struct FusedParams {
 // The split kernels: limbs of 2^17 words, one per workgroup, transformed as
 // two 2^16-word halves in turn.
 constexpr int FUSED_SPLIT_LOG_N = 17;
+// The split inverse with on-the-fly twiddles stages tw[0 .. 2047] and the
+// 2048-word msb appendix of the 2^17-word table after the exchange buffer
+// (16 KiB; 148 KiB of the CU's 160 KiB in all).
+constexpr uint32_t FUSED_SPLIT_TWL_WORDS = 2048 + 2048;
+constexpr uint32_t fused_split_lds_bytes(bool ot) {
+    return (FUSED_LDS_WORDS + (ot ? FUSED_SPLIT_TWL_WORDS : FUSED_TWL_WORDS)) * 4u;
+}
 
 }  // namespace ntt

@@ -189,6 +189,11 @@ static FusedVariant variant_of(const TunedEntry* t, int logN, bool inverse, bool
         v.eb = 6;
         v.logw = 16;
     }
+    if (logN == FUSED_SPLIT_LOG_N) {   // the split kernels: width 6; on-the-fly twiddles for the inverse only
+        v.eb = 6;
+        v.logw = 16;
+        v.ot = v.ot && inverse;
+    }
     return v;
 }
 
@@ -215,6 +220,11 @@ void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, FusedVari
     if (v->ot) {   // the on-the-fly twiddle kernels: 1024 x 64 words per CU
         v->eb = 6;
         v->logw = 16;
+    }
+    if (P->logN == FUSED_SPLIT_LOG_N) {   // the split kernels: width 6; on-the-fly twiddles for the inverse only
+        v->eb = 6;
+        v->logw = 16;
+        v->ot = v->ot && inverse;
     }
 }
 

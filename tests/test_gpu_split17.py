@@ -1,10 +1,12 @@
-"""GPU parity of the split fused kernels: limbs of 2^17 words as two 2^16-word
-halves, one workgroup each (the forward computes the cross-half stage on
-load; the inverse pairs the limb's two workgroups for its last stage).
-Bit-exact against the oracle (integer arithmetic: tolerance 0): both
-arithmetics (lazy, canonical), both domains, limb counts 1..8, several
-operands per call, repeated launches (the pair words return to zero),
-out-of-place forms, the prologue/epilogue constants, and hipGraph replays."""
+"""GPU parity of the split fused kernels: a limb of 2^17 words per workgroup,
+transformed as two 2^16-word halves in turn (the forward computes the
+cross-half stage on load and parks half 1 in the output; the inverse parks
+half 0's result there and combines it with half 1's).  Bit-exact against the
+oracle (integer arithmetic: tolerance 0): both arithmetics (lazy, canonical),
+both domains, limb counts 1..8, several operands per call, repeated launches,
+edge values, the prologue/epilogue constants, hipGraph replays, and agreement
+with the two-pass engine at 48 limbs.  The out-of-place forms at logN 17 run
+in tests/test_gpu_moddown.py on both engines."""
 import numpy as np
 import pytest
 
@@ -53,8 +55,7 @@ def test_split_matches_oracle(env, limbs, count, arith, domain):
     plan.inverse(A, count)
     gu.sync()
     assert np.array_equal(gu.to_host(A).reshape(xs.shape), xs)
-    # the inverse on its own input, three launches in a row (each one leaves
-    # the pair words at zero for the next)
+    # the inverse on its own input, three launches in a row
     for r in range(3):
         ys = np.stack([gu.random_operand(LOGN, moduli, 900 + 13 * k + r) for k in range(count)])
         B = gu.to_dev(ys)
@@ -141,7 +142,7 @@ def test_split_graph_replays(env):
     ctx, plan = _plan(ntt_amd, moduli, 1)
     src, A, B = gu.to_dev(x), gu.to_dev(x), gu.to_dev(y)
     s = ntt_amd.Stream()
-    plan.inverse(B, 1, s)          # pair words allocated before the capture
+    plan.inverse(B, 1, s)
     s.synchronize()
     exp_inv = orc.inverse(y, LOGN, moduli, 1)
     assert np.array_equal(gu.to_host(B).reshape(y.shape), exp_inv)
@@ -188,3 +189,30 @@ def test_split_agrees_with_two_pass_many_operands(env):
         assert np.array_equal(gu.to_host(A).reshape(xs.shape), xs), engine
     assert np.array_equal(outs[0], outs[1])
     assert np.array_equal(outs[0][1], orc.forward(xs[1], LOGN, moduli, 1))
+
+
+@pytest.mark.parametrize("arith", [1, 0])
+@pytest.mark.parametrize("domain", [1, 0])
+def test_split_inverse_on_the_fly_twiddles(env, arith, domain):
+    """The split inverse with on-the-fly twiddles (tw[i] = tw[i % 64] msb[i / 64]
+    on the 2^17-word table, staged in LDS); the forward stays on table twiddles."""
+    ntt_amd, gu, orc = env
+    limbs, count = 5, 2
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(LOGN, limbs)]
+    ctx, plan = _plan(ntt_amd, moduli, domain, arith)
+    plan.set_twiddle_mode(ntt_amd.TWIDDLE_OT)
+    assert plan.fused_variant_full(count, ntt_amd.INVERSE)[3] is True
+    assert plan.fused_variant_full(count, ntt_amd.FORWARD)[3] is False
+    ys = np.stack([gu.random_operand(LOGN, moduli, 40 + k + 10 * arith + domain) for k in range(count)])
+    B = gu.to_dev(ys)
+    _print("case split OT inverse", arith, domain)
+    for r in range(2):      # two tables: the staged words are refreshed per limb
+        B.upload(ys)
+        plan.inverse(B, count)
+        gu.sync()
+        got = gu.to_host(B).reshape(ys.shape)
+        for k in range(count):
+            assert np.array_equal(got[k], orc.inverse(ys[k], LOGN, moduli, domain)), (r, k)
+    plan.forward(B, count)
+    gu.sync()
+    assert np.array_equal(gu.to_host(B).reshape(ys.shape), ys)
