@@ -358,15 +358,19 @@ def bench_c3(ntt_amd, dev, stream, traffic_path: str, tune: bool = True, count: 
     inv = _timed(dev, stream, lambda: plan.inverse(pool, count, stream), reps)
     ok = bool(np.array_equal(before, pool.numpy(np.uint32, nbytes=4 * words)))
     bfly = 0.5 * N * logN * limbs * count
-    eng = {1: "two_pass", 2: "fused"}.get(plan.engine(count), "?")
-    if eng == "fused":
-        # one kernel per plan launch (the split kernels, one limb per workgroup)
-        ebf, lzf, lwf, otf = plan.fused_variant_full(count, ntt_amd.FORWARD)
-        ebi, lzi, lwi, oti = plan.fused_variant_full(count, ntt_amd.INVERSE)
-        kms = {fused_kernel_name(logN, False, ebf, lzf, lwf, otf): fwd,
-               fused_kernel_name(logN, True, ebi, lzi, lwi, oti): inv}
-    else:
-        kms = _pass_kernel_ms(dev, stream, plan, pool, count, cfg, all(q < (1 << 30) for q, _ in moduli))
+    engs = [{1: "two_pass", 2: "fused"}.get(plan.engine(count, d), "?") for d in (ntt_amd.FORWARD, ntt_amd.INVERSE)]
+    eng = engs[0] if engs[0] == engs[1] else f"forward {engs[0]}, inverse {engs[1]}"
+    # kernel times per direction: one kernel per plan launch on the fused
+    # engine (the split kernels, one limb per workgroup), the two passes on
+    # the two-pass engine
+    tp = _pass_kernel_ms(dev, stream, plan, pool, count, cfg, all(q < (1 << 30) for q, _ in moduli))
+    kms = {}
+    for d, t in ((0, fwd), (1, inv)):
+        if engs[d] == "fused":
+            eb, lz, lw, ot = plan.fused_variant_full(count, d)
+            kms[fused_kernel_name(logN, d == 1, eb, lz, lw, ot)] = t
+        else:
+            kms.update({k: v for k, v in tp.items() if (k.split(", ")[1] == "true") == (d == 1)})
     step = fwd + inv
     out = {"workload": f"N=2^17, 48 limbs (48 largest primes < 2^30, q = 1 mod 2^18), Montgomery, {count} operands "
                        f"({4 * words * count / 2**30:.2f} GiB), forward then inverse, seed 0",

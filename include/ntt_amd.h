@@ -245,6 +245,10 @@ int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction
                                int* log2_words, int* on_the_fly);
 /* Engine a launch of `count` operands would use (AUTO resolved). */
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count);
+/* Engine a launch of `count` operands would use in `direction` (the
+ * autotuner picks each direction's engine; ntt_plan_get_engine reports the
+ * forward's). */
+int ntt_plan_get_engine_for(const ntt_plan* plan, size_t count, int direction);
 int ntt_plan_forward(ntt_plan* plan, uint32_t* A_dev, void* stream);
 int ntt_plan_inverse(ntt_plan* plan, uint32_t* A_dev, void* stream);
 /* Same transform over `count` operands laid out back to back
@@ -361,6 +365,7 @@ typedef struct {
     int fused_words_log2_inv;   /* fused inverse words per workgroup (log2) */
     int fused_ot;               /* fused forward twiddles: 1 on the fly, 0 table */
     int fused_ot_inv;           /* fused inverse twiddles: 1 on the fly, 0 table */
+    int engine_inv;             /* ntt_engine chosen for the inverse (engine: the forward's) */
 } ntt_tune_result;
 
 int ntt_autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_t limbs,

@@ -130,7 +130,8 @@ class TuneResult(ctypes.Structure):
                 ("fused_fwd_ms", ctypes.c_double), ("fused_inv_ms", ctypes.c_double), ("fused_stagger", ctypes.c_int),
                 ("fused_eb", ctypes.c_int), ("fused_eb_inv", ctypes.c_int), ("fused_lazy", ctypes.c_int),
                 ("fused_lazy_inv", ctypes.c_int), ("fused_words_log2", ctypes.c_int),
-                ("fused_words_log2_inv", ctypes.c_int), ("fused_ot", ctypes.c_int), ("fused_ot_inv", ctypes.c_int)]
+                ("fused_words_log2_inv", ctypes.c_int), ("fused_ot", ctypes.c_int), ("fused_ot_inv", ctypes.c_int),
+                ("engine_inv", ctypes.c_int)]
 
     def as_dict(self):
         return {"cfg": self.cfg.astuple(), "chunk": int(self.chunk), "fwd_ms": self.fwd_ms, "inv_ms": self.inv_ms,
@@ -140,7 +141,8 @@ class TuneResult(ctypes.Structure):
                 "fused_stagger": self.fused_stagger, "fused_eb": self.fused_eb, "fused_eb_inv": self.fused_eb_inv,
                 "fused_lazy": self.fused_lazy, "fused_lazy_inv": self.fused_lazy_inv,
                 "fused_words_log2": self.fused_words_log2, "fused_words_log2_inv": self.fused_words_log2_inv,
-                "fused_ot": self.fused_ot, "fused_ot_inv": self.fused_ot_inv}
+                "fused_ot": self.fused_ot, "fused_ot_inv": self.fused_ot_inv,
+                "engine_inv": {1: "two_pass", 2: "fused"}.get(self.engine_inv, self.engine_inv)}
 
 
 class BatchJob(ctypes.Structure):
@@ -251,6 +253,7 @@ _PROTOS = {
     "ntt_plan_get_fused_variant": (_I, [_P, _SZ, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
                                         ctypes.POINTER(_I)]),
     "ntt_plan_get_engine": (_I, [_P, _SZ]),
+    "ntt_plan_get_engine_for": (_I, [_P, _SZ, _I]),
     "ntt_autotune": (_I, [_P, _I, ctypes.POINTER(ModulusConfig), _SZ, _SZ, _I, _I, ctypes.POINTER(TuneResult)]),
     "ntt_context_set_tuning": (_I, [_P, _I, _SZ, _PCFG, _SZ, _I]),
     "ntt_context_clear_tuning": (_I, [_P]),
@@ -815,8 +818,9 @@ class Plan:
         """Fused workgroup shape, both directions: 7 (512 threads x 128 words), 6 (1024 x 64), 0 = tuned."""
         _check(_lib.ntt_plan_set_fused_shape(self.handle, eb), "ntt_plan_set_fused_shape")
 
-    def engine(self, count: int = 1) -> int:
-        return int(_lib.ntt_plan_get_engine(self.handle, count))
+    def engine(self, count: int = 1, direction: int = FORWARD) -> int:
+        """Engine a launch of `count` operands uses in `direction` (1 two-pass, 2 fused)."""
+        return int(_lib.ntt_plan_get_engine_for(self.handle, count, direction))
 
     def run_pass(self, A, count: int, direction: int, pass_index: int, stream=None):
         _check(_lib.ntt_plan_run_pass(self.handle, _addr(A), count, direction, pass_index, _stream(stream)),

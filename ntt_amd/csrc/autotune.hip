@@ -181,7 +181,7 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
 
     size_t best_chunk = 0;
     double def_fwd = 0, def_inv = 0, fused_fwd = 0, fused_inv = 0;
-    int engine = NTT_ENGINE_TWO_PASS;
+    int engine = NTT_ENGINE_TWO_PASS, engine_inv = NTT_ENGINE_TWO_PASS;
     FusedVariant vf, vi;   // fused variants per direction
     int stagger = 0;
     if (rc == NTT_OK) {
@@ -251,12 +251,17 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
                     best_inv = i;
                 }
             }
-            if (rc == NTT_OK && bf + bi < best_fwd + best_inv) {
+            // each direction on the engine that measured faster (the
+            // chunking applies to two-pass launches only)
+            if (rc == NTT_OK && bf < best_fwd) {
                 engine = NTT_ENGINE_FUSED;
                 best_fwd = bf;
-                best_inv = bi;
-                best_chunk = 0;
             }
+            if (rc == NTT_OK && bi < best_inv) {
+                engine_inv = NTT_ENGINE_FUSED;
+                best_inv = bi;
+            }
+            if (engine == NTT_ENGINE_FUSED && engine_inv == NTT_ENGINE_FUSED) best_chunk = 0;
         }
         // the default config, for comparison
         if (rc == NTT_OK) {
@@ -278,6 +283,7 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
     TunedEntry e;
     e.cfg = best;
     e.engine = engine;
+    e.engine_inv = engine_inv;
     e.fused_eb_fwd = vf.eb;
     e.fused_eb_inv = vi.eb;
     e.fused_lazy = vf.lazy;
@@ -303,6 +309,7 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
         out->default_inv_ms = def_inv;
         out->candidates = ncand;
         out->engine = engine;
+        out->engine_inv = engine_inv;
         out->fused_fwd_ms = fused_fwd;
         out->fused_inv_ms = fused_inv;
         out->fused_stagger = stagger;
@@ -341,6 +348,7 @@ int ntt_context_set_tuning(ntt_context* ctx, int logN, size_t total_limbs, const
     e.cfg = *cfg;
     e.chunk = chunk;
     e.engine = engine == NTT_ENGINE_FUSED ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
+    e.engine_inv = e.engine;
     std::lock_guard<std::mutex> g(ctx->tune_mu);
     ctx->tuned[std::make_pair(logN, log2_bucket(total_limbs))] = e;
     return NTT_OK;
@@ -360,17 +368,18 @@ int ntt_context_save_tuning(const ntt_context* ctx, const char* path) {
     FILE* f = std::fopen(path, "w");
     if (!f) return NTT_ERR_INVALID_ARGUMENT;
     std::lock_guard<std::mutex> g(ctx->tune_mu);
-    std::fprintf(f, "# ntt_amd tuning v4: logN limbs_log2 chunk r s w x4 fwd_ms inv_ms engine fused_eb_fwd "
+    std::fprintf(f, "# ntt_amd tuning v5: logN limbs_log2 chunk r s w x4 fwd_ms inv_ms engine fused_eb_fwd "
                     "fused_eb_inv fused_lazy stagger fused_lazy_inv fused_logw_fwd fused_logw_inv fused_ot_fwd "
-                    "fused_ot_inv\n");
+                    "fused_ot_inv engine_inv\n");
     for (auto& kv : ctx->tuned) {
         const TunedEntry& e = kv.second;
         const ntt_phase_config* p = &e.cfg.ntt_phase1;
         std::fprintf(f, "%d %d %zu", kv.first.first, kv.first.second, e.chunk);
         for (int i = 0; i < 4; ++i) std::fprintf(f, " %d %d %d", p[i].radix_stages, p[i].stage_merging, p[i].warp_batching);
-        std::fprintf(f, " %.6f %.6f %d %d %d %d %d %d %d %d %d %d\n", e.fwd_ms, e.inv_ms, e.engine, e.fused_eb_fwd,
-                     e.fused_eb_inv, e.fused_lazy ? 1 : 0, e.stagger, e.fused_lazy_inv ? 1 : 0, e.fused_logw_fwd,
-                     e.fused_logw_inv, e.fused_ot_fwd ? 1 : 0, e.fused_ot_inv ? 1 : 0);
+        std::fprintf(f, " %.6f %.6f %d %d %d %d %d %d %d %d %d %d %d\n", e.fwd_ms, e.inv_ms, e.engine,
+                     e.fused_eb_fwd, e.fused_eb_inv, e.fused_lazy ? 1 : 0, e.stagger, e.fused_lazy_inv ? 1 : 0,
+                     e.fused_logw_fwd, e.fused_logw_inv, e.fused_ot_fwd ? 1 : 0, e.fused_ot_inv ? 1 : 0,
+                     e.engine_inv);
     }
     std::fclose(f);
     return NTT_OK;
@@ -388,11 +397,13 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         size_t chunk;
         double fm = 0, im = 0;
         int eng = NTT_ENGINE_TWO_PASS, febf = FUSED_DEFAULT_EB_FWD, febi = FUSED_DEFAULT_EB_INV, lz = 1, stag = 0;
-        int lzi = 1, lwf = 16, lwi = 16, otf = 0, oti = 0;
+        int lzi = 1, lwf = 16, lwi = 16, otf = 0, oti = 0, engi = -1;
         int n = std::sscanf(line,
-                            "%d %d %zu %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d %d %d %d %d %d %d %d %d %d",
+                            "%d %d %zu %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf %d %d %d %d %d %d %d %d %d %d %d",
                             &logN, &lb, &chunk, &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9],
-                            &v[10], &v[11], &fm, &im, &eng, &febf, &febi, &lz, &stag, &lzi, &lwf, &lwi, &otf, &oti);
+                            &v[10], &v[11], &fm, &im, &eng, &febf, &febi, &lz, &stag, &lzi, &lwf, &lwi, &otf, &oti,
+                            &engi);
+        if (engi < 0) engi = eng;   // files of one engine for both directions
         if (n < 15) { rc = NTT_ERR_INVALID_ARGUMENT; break; }
         TunedEntry e;
         ntt_phase_config* p = &e.cfg.ntt_phase1;
@@ -400,6 +411,7 @@ int ntt_context_load_tuning(ntt_context* ctx, const char* path) {
         if (validate_config(logN, &e.cfg, NTT_LDS_BYTES_GFX950) != NTT_OK) { rc = NTT_ERR_CONFIG; break; }
         e.chunk = chunk;
         e.engine = (eng == NTT_ENGINE_FUSED && fused_plan_supported(logN)) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
+        e.engine_inv = (engi == NTT_ENGINE_FUSED && fused_plan_supported(logN)) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
         e.fused_eb_fwd = febf == 6 ? 6 : 7;
         e.fused_eb_inv = febi == 6 ? 6 : 7;
         e.fused_lazy = lz != 0;

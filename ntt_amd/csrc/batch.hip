@@ -236,7 +236,8 @@ int ntt_batch_create(ntt_context* ctx, const ntt_batch_job* jobs, size_t njobs, 
         size_t total_limbs = 0;
         for (size_t i : it->second) total_limbs += b->jobs[i].limbs;
         TunedEntry t;
-        if (ctx->tuned_lookup(std::get<0>(it->first), total_limbs, &t) && t.engine == NTT_ENGINE_TWO_PASS) {
+        if (ctx->tuned_lookup(std::get<0>(it->first), total_limbs, &t) &&
+            t.engine_for(std::get<1>(it->first) != 0) == NTT_ENGINE_TWO_PASS) {
             for (size_t i : it->second) {
                 const ntt_batch_job& j = b->jobs[i];
                 tp_shape[std::make_tuple(j.logN, j.direction == NTT_INVERSE ? 1 : 0, j.domain,

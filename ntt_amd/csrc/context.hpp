@@ -44,7 +44,9 @@ struct LimbMap {
 // Autotuner result for one (logN, words per launch) bucket.
 struct TunedEntry {
     ntt_transform_config cfg;
-    int engine = NTT_ENGINE_TWO_PASS;   // NTT_ENGINE_TWO_PASS or NTT_ENGINE_FUSED
+    int engine = NTT_ENGINE_TWO_PASS;   // NTT_ENGINE_TWO_PASS or NTT_ENGINE_FUSED (forward)
+    int engine_inv = NTT_ENGINE_TWO_PASS;   // the inverse's
+    int engine_for(bool inverse) const { return inverse ? engine_inv : engine; }
     int fused_eb_fwd = FUSED_DEFAULT_EB_FWD, fused_eb_inv = FUSED_DEFAULT_EB_INV;   // fused window width (6 or 7)
     bool fused_lazy = true;             // fused forward arithmetic: lazy (when every q < 2^30)
     bool fused_lazy_inv = true;         // fused inverse arithmetic: lazy (when every q < 2^30)
@@ -110,8 +112,9 @@ int plan_build(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, siz
 // chunking.
 TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool inverse);
 int log2_bucket(size_t v);
-// Engine a launch of `count` operands runs on (resolves NTT_ENGINE_AUTO).
-int plan_engine(const ntt_plan* P, size_t count);
+// Engine a launch of `count` operands runs on (resolves NTT_ENGINE_AUTO), per
+// direction (the autotuner picks each direction's engine).
+int plan_engine(const ntt_plan* P, size_t count, bool inverse = false);
 int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse, hipStream_t stream,
                       const FusedVariant& v, int stagger, const uint32_t* src = nullptr, const uint32_t* sub = nullptr,
                       const LimbMap& map = LimbMap());

@@ -165,10 +165,10 @@ TransformLaunch plan_launch(const ntt_plan* P, uint32_t* A, size_t count, bool i
     return L;
 }
 
-int plan_engine(const ntt_plan* P, size_t count) {
+int plan_engine(const ntt_plan* P, size_t count, bool inverse) {
     if (P->engine != NTT_ENGINE_AUTO) return P->engine;
     TunedEntry t;
-    if (!P->cfg_fixed && P->ctx->tuned_lookup(P->logN, P->limbs * count, &t)) return t.engine;
+    if (!P->cfg_fixed && P->ctx->tuned_lookup(P->logN, P->limbs * count, &t)) return t.engine_for(inverse);
     if (P->cfg_fixed) return NTT_ENGINE_TWO_PASS;
     return fused_plan_supported(P->logN) ? NTT_ENGINE_FUSED : NTT_ENGINE_TWO_PASS;
 }
@@ -341,7 +341,7 @@ static int plan_run(ntt_plan* plan, uint32_t* A_dev, size_t count, bool inverse,
     if (sub && (sub == A_dev || overlap(sub, sub_bytes, A_dev, bytes))) return NTT_ERR_INVALID_ARGUMENT;
     // the split (2^17-word) fused kernels have no mod-down form: at that size
     // it runs on the two-pass kernels
-    if (plan_engine(plan, count) == NTT_ENGINE_FUSED && !(sub && plan->logN == FUSED_SPLIT_LOG_N)) {
+    if (plan_engine(plan, count, inverse) == NTT_ENGINE_FUSED && !(sub && plan->logN == FUSED_SPLIT_LOG_N)) {
         int stagger = 0;
         FusedVariant v;
         plan_fused_variant(plan, count, inverse, &v, &stagger);
@@ -452,6 +452,11 @@ int ntt_plan_get_fused_variant(const ntt_plan* plan, size_t count, int direction
 int ntt_plan_get_engine(const ntt_plan* plan, size_t count) {
     if (!plan || count == 0) return -NTT_ERR_INVALID_ARGUMENT;
     return plan_engine(plan, count);
+}
+
+int ntt_plan_get_engine_for(const ntt_plan* plan, size_t count, int direction) {
+    if (!plan || count == 0 || (direction != NTT_FORWARD && direction != NTT_INVERSE)) return -NTT_ERR_INVALID_ARGUMENT;
+    return plan_engine(plan, count, direction == NTT_INVERSE);
 }
 
 int ntt_plan_run_pass(ntt_plan* plan, uint32_t* A_dev, size_t count, int direction, int pass, void* stream) {

@@ -273,7 +273,7 @@ int enqueue_launch(ntt_server* S, Slot& s, const std::vector<std::shared_ptr<Req
         // the engine the autotuner measured faster for a launch of this size
         // (fused when the size is untuned)
         TunedEntry tuned;
-        const bool tuned_two_pass = S->ctx->tuned_lookup(n, tot, &tuned) && tuned.engine == NTT_ENGINE_TWO_PASS;
+        const bool tuned_two_pass = S->ctx->tuned_lookup(n, tot, &tuned) && tuned.engine_for(inv) == NTT_ENGINE_TWO_PASS;
         if (fused_plan_supported(n) && !tuned_two_pass) {
             const size_t first = nd;
             const FusedVariant v = fused_variant_for(S->ctx, n, tot, inv, std::get<2>(kv.first) != 0);

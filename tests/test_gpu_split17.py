@@ -216,3 +216,36 @@ def test_split_inverse_on_the_fly_twiddles(env, arith, domain):
     plan.forward(B, count)
     gu.sync()
     assert np.array_equal(gu.to_host(B).reshape(ys.shape), ys)
+
+
+def test_tuned_engine_per_direction(env, tmp_path):
+    """The autotuner picks each direction's engine (split or two-pass); plans
+    follow it per direction, results stay exact, and the choice survives a
+    save/load of the tuning file."""
+    ntt_amd, gu, orc = env
+    limbs, count = 6, 4
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(LOGN, limbs)]
+    ctx = ntt_amd.Context(0)
+    _print("case tuned engines 17")
+    r = ctx.autotune(LOGN, moduli, count=count, reps=2)
+    eng = {"two_pass": ntt_amd.ENGINE_TWO_PASS, "fused": ntt_amd.ENGINE_FUSED}
+    plan = ctx.plan(LOGN, moduli, 1)
+    assert plan.engine(count, ntt_amd.FORWARD) == eng[r["engine"]]
+    assert plan.engine(count, ntt_amd.INVERSE) == eng[r["engine_inv"]]
+    xs = np.stack([gu.random_operand(LOGN, moduli, 60 + k) for k in range(count)])
+    A = gu.to_dev(xs)
+    plan.forward(A, count)
+    gu.sync()
+    got = gu.to_host(A).reshape(xs.shape)
+    for k in range(count):
+        assert np.array_equal(got[k], orc.forward(xs[k], LOGN, moduli, 1)), k
+    plan.inverse(A, count)
+    gu.sync()
+    assert np.array_equal(gu.to_host(A).reshape(xs.shape), xs)
+    path = str(tmp_path / "tune17.txt")
+    ctx.save_tuning(path)
+    ctx2 = ntt_amd.Context(0)
+    ctx2.load_tuning(path)
+    plan2 = ctx2.plan(LOGN, moduli, 1)
+    for d in (ntt_amd.FORWARD, ntt_amd.INVERSE):
+        assert plan2.engine(count, d) == plan.engine(count, d)
