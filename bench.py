@@ -729,7 +729,9 @@ def main():
     log("kernel ms: " + ", ".join(f"{n} {v:.4f}" for n, v in kernel_ms.items()))
 
     # the binding compute roof: the same butterfly code on register-resident operands
-    inverse_dom = dom.split("<", 1)[1].split(",")[1].strip() == "true"   # second template argument: INV
+    # INV: the first template argument of the split kernels, the second of the others
+    targs = dom.split("<", 1)[1].split(",")
+    inverse_dom = targs[0 if dom.startswith("fused_split_kernel") else 1].strip().rstrip(">") == "true"
     lazy_dom = (lazy and engine_used != "fused") or (engine_used == "fused" and (lzi if inverse_dom else lzf))
     variant = (dev.BUTTERFLY_GS if inverse_dom else dev.BUTTERFLY_CT) + (1 if lazy_dom else 0)
     roof_bfly = dev.butterfly_roof(variant, stream)
