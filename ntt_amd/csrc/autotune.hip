@@ -218,9 +218,8 @@ int autotune(ntt_context* ctx, int logN, const ntt_modulus_config* moduli, size_
                     for (int ot = 0; rc == NTT_OK && ot <= (eb == 6 && logw == 16 ? 1 : 0); ++ot)
                     for (int lz = P.lazy ? 1 : 0; rc == NTT_OK && lz >= 0; --lz)
                         for (int inv = 0; rc == NTT_OK && inv < 2; ++inv) {
-                            // split limbs (2^17 words): width 6, on-the-fly twiddles
-                            // for the inverse only
-                            if (logN == FUSED_SPLIT_LOG_N && (eb != 6 || (ot && !inv))) continue;
+                            // split limbs (2^17 words): width 6
+                            if (logN == FUSED_SPLIT_LOG_N && eb != 6) continue;
                             FusedVariant v;
                             v.eb = eb;
                             v.lazy = lz != 0;

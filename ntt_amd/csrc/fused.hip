@@ -83,10 +83,9 @@ int fused_prepare() {
                                                    hipFuncAttributeMaxDynamicSharedMemorySize, lds15);
             }
         for (int d = 0; d < 8 && attr_err == hipSuccess; ++d)
-            if (!(d & 4) || (d & 1))   // on-the-fly twiddles: inverse only
-                attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn_split(d & 1, d & 2, d & 4)),
-                                               hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               (int)fused_split_lds_bytes(d & 4));
+            attr_err = hipFuncSetAttribute(reinterpret_cast<const void*>(fused_fn_split(d & 1, d & 2, d & 4)),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)fused_split_lds_bytes(d & 1, d & 4));
         for (int m = FUSED_MIN_LOG_N; m <= FUSED_MAX_LOG_N && attr_err == hipSuccess; ++m)
             for (int lz = 0; lz < 2 && attr_err == hipSuccess; ++lz) {
                 FusedFn f = fused_fn_md(m, lz == 1);
@@ -122,11 +121,9 @@ int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipSt
     if (p.total_items == 0) return NTT_OK;
     if (eb != 6 && eb != 7) return NTT_ERR_INVALID_ARGUMENT;
     if (n == FUSED_SPLIT_LOG_N) {
-        // split limbs: width 6, one limb per workgroup; on-the-fly twiddles
-        // for the inverse only
-        const bool sot = ot && inv;
-        return launch_fused_fn(fused_fn_split(inv, lazy, sot), 6, p, stream, grid_limit, 16,
-                               fused_split_lds_bytes(sot));
+        // split limbs: width 6, one limb per workgroup
+        return launch_fused_fn(fused_fn_split(inv, lazy, ot), 6, p, stream, grid_limit, 16,
+                               fused_split_lds_bytes(inv, ot));
     }
     if (ot) {
         // on-the-fly twiddles: tables with the msb appendix, 1024 x 64 words per CU
@@ -182,7 +179,7 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
         // split limbs: table twiddles, width 6; no mod-down form
         if (sub) return NTT_ERR_CONFIG;
         if (p.total_items == 0) return NTT_OK;
-        return launch_fused(P->logN, inverse, 6, v.lazy && P->lazy, p, stream, 0, 16, v.ot && inverse);
+        return launch_fused(P->logN, inverse, 6, v.lazy && P->lazy, p, stream, 0, 16, v.ot);
     }
     if (v.ot && !sub) {
         // on-the-fly twiddles: 1024 x 64 words, lazy when every q < 2^30 (and

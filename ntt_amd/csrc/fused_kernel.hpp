@@ -211,7 +211,9 @@ __device__ __forceinline__ void stages(uint32_t (&v)[1 << EB], gcu32* __restrict
             if constexpr (OTS) {
                 // the msb appendix follows the table (of 2^(n+1) words for a split limb)
                 constexpr uint32_t MSB = 1u << (SPLIT ? n + 1 : n);
-                ot_msb = TL ? twl[FUSED_TWL_TW + (TB >> (OT_LSB_BITS - k))] : tw[MSB + (TB >> (OT_LSB_BITS - k))];
+                // the staged msb words follow the staged table words
+                constexpr uint32_t TWL_TW = SPLIT ? fused_split_twl_tw(INV) : FUSED_TWL_TW;
+                ot_msb = TL ? twl[TWL_TW + (TB >> (OT_LSB_BITS - k))] : tw[MSB + (TB >> (OT_LSB_BITS - k))];
                 ot_lo = (TB & ((1u << (OT_LSB_BITS - k)) - 1)) << k;
             }
             static_for<E / 2>([&](auto P) __attribute__((always_inline)) {
@@ -398,8 +400,7 @@ __device__ __forceinline__ void split_inverse_store(uint32_t (&v)[1 << EB], cons
 template <int n, bool INV, bool LAZY, int EB, bool MD, bool OT, bool TWL = false, bool SPLIT = false>
 __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_limb, uint32_t* lds,
                                          const uint32_t** staged) {
-    static_assert(!SPLIT || (n == 16 && EB == 6 && !MD && (!OT || INV)),
-                  "split limbs: 2 x 2^16 words, width 6, on-the-fly twiddles for the inverse only");
+    static_assert(!SPLIT || (n == 16 && EB == 6 && !MD), "split limbs: 2 x 2^16 words, width 6");
     constexpr int NL = SPLIT ? n + 1 : n;   // log2 of the limb stride
     constexpr int E = 1 << EB;
     constexpr int G_BITS = n - EB;
@@ -440,12 +441,12 @@ __device__ __forceinline__ void run_item(const FusedJob& job, uint32_t first_lim
         static_assert(n == 16 && OT, "staged twiddles: one 2^16-word limb per workgroup, on-the-fly twiddles");
         if ((const uint32_t*)tw != *staged) {   // uniform: the table base is in SGPRs
             __syncthreads();   // the previous item's rounds have read the staged words
-            // a split limb's table has 2^(n+1) words and an appendix of 2^(n-5)
-            // (its inverse's later rounds read tw[0 .. 2047] as well)
-            constexpr uint32_t MSB_WORDS = SPLIT ? FUSED_SPLIT_TWL_WORDS - FUSED_TWL_TW : FUSED_TWL_WORDS - FUSED_TWL_TW;
-            for (uint32_t i = threadIdx.x; i < FUSED_TWL_TW; i += blockDim.x) twl[i] = tw[i];
-            for (uint32_t i = threadIdx.x; i < MSB_WORDS; i += blockDim.x)
-                twl[FUSED_TWL_TW + i] = tw[(1u << NL) + i];
+            // a split limb's table has 2^(n+1) words and an appendix of 2^(n-5);
+            // its later rounds read tw[0 .. 2047] (inverse) or tw[0 .. 4095]
+            constexpr uint32_t TWL_TW = SPLIT ? fused_split_twl_tw(INV) : FUSED_TWL_TW;
+            constexpr uint32_t MSB_WORDS = SPLIT ? FUSED_SPLIT_MSB_WORDS : FUSED_TWL_WORDS - FUSED_TWL_TW;
+            for (uint32_t i = threadIdx.x; i < TWL_TW; i += blockDim.x) twl[i] = tw[i];
+            for (uint32_t i = threadIdx.x; i < MSB_WORDS; i += blockDim.x) twl[TWL_TW + i] = tw[(1u << NL) + i];
             __syncthreads();
             *staged = (const uint32_t*)tw;
         }

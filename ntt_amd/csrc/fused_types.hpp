@@ -114,12 +114,14 @@ struct FusedParams {
 // The split kernels: limbs of 2^17 words, one per workgroup, transformed as
 // two 2^16-word halves in turn.
 constexpr int FUSED_SPLIT_LOG_N = 17;
-// The split inverse with on-the-fly twiddles stages tw[0 .. 2047] and the
-// 2048-word msb appendix of the 2^17-word table after the exchange buffer
-// (16 KiB; 148 KiB of the CU's 160 KiB in all).
-constexpr uint32_t FUSED_SPLIT_TWL_WORDS = 2048 + 2048;
-constexpr uint32_t fused_split_lds_bytes(bool ot) {
-    return (FUSED_LDS_WORDS + (ot ? FUSED_SPLIT_TWL_WORDS : FUSED_TWL_WORDS)) * 4u;
+// The split kernels with on-the-fly twiddles stage the table words their
+// later rounds read, tw[0 .. 2047] (inverse) or tw[0 .. 4095] (forward), and
+// the 2048-word msb appendix of the 2^17-word table after the exchange buffer
+// (16 or 24 KiB; 148 or 156 KiB of the CU's 160 KiB in all).
+constexpr uint32_t FUSED_SPLIT_MSB_WORDS = 2048;
+constexpr uint32_t fused_split_twl_tw(bool inv) { return inv ? 2048u : 4096u; }
+constexpr uint32_t fused_split_lds_bytes(bool inv, bool ot) {
+    return (FUSED_LDS_WORDS + (ot ? fused_split_twl_tw(inv) + FUSED_SPLIT_MSB_WORDS : FUSED_TWL_WORDS)) * 4u;
 }
 
 }  // namespace ntt

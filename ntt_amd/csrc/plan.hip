@@ -189,10 +189,9 @@ static FusedVariant variant_of(const TunedEntry* t, int logN, bool inverse, bool
         v.eb = 6;
         v.logw = 16;
     }
-    if (logN == FUSED_SPLIT_LOG_N) {   // the split kernels: width 6; on-the-fly twiddles for the inverse only
+    if (logN == FUSED_SPLIT_LOG_N) {   // the split kernels: width 6
         v.eb = 6;
         v.logw = 16;
-        v.ot = v.ot && inverse;
     }
     return v;
 }
@@ -221,10 +220,9 @@ void plan_fused_variant(const ntt_plan* P, size_t count, bool inverse, FusedVari
         v->eb = 6;
         v->logw = 16;
     }
-    if (P->logN == FUSED_SPLIT_LOG_N) {   // the split kernels: width 6; on-the-fly twiddles for the inverse only
+    if (P->logN == FUSED_SPLIT_LOG_N) {   // the split kernels: width 6
         v->eb = 6;
         v->logw = 16;
-        v->ot = v->ot && inverse;
     }
 }
 

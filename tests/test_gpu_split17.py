@@ -193,29 +193,36 @@ def test_split_agrees_with_two_pass_many_operands(env):
 
 @pytest.mark.parametrize("arith", [1, 0])
 @pytest.mark.parametrize("domain", [1, 0])
-def test_split_inverse_on_the_fly_twiddles(env, arith, domain):
-    """The split inverse with on-the-fly twiddles (tw[i] = tw[i % 64] msb[i / 64]
-    on the 2^17-word table, staged in LDS); the forward stays on table twiddles."""
+def test_split_on_the_fly_twiddles(env, arith, domain):
+    """The split kernels with on-the-fly twiddles (tw[i] = tw[i % 64] msb[i / 64]
+    on the 2^17-word table, the later rounds' words staged in LDS), both
+    directions, several limbs and operands (the staged words are refreshed per
+    limb's table)."""
     ntt_amd, gu, orc = env
     limbs, count = 5, 2
     moduli = [tuple(m) for m in ntt_amd.generate_moduli(LOGN, limbs)]
     ctx, plan = _plan(ntt_amd, moduli, domain, arith)
     plan.set_twiddle_mode(ntt_amd.TWIDDLE_OT)
-    assert plan.fused_variant_full(count, ntt_amd.INVERSE)[3] is True
-    assert plan.fused_variant_full(count, ntt_amd.FORWARD)[3] is False
-    ys = np.stack([gu.random_operand(LOGN, moduli, 40 + k + 10 * arith + domain) for k in range(count)])
-    B = gu.to_dev(ys)
-    _print("case split OT inverse", arith, domain)
-    for r in range(2):      # two tables: the staged words are refreshed per limb
-        B.upload(ys)
-        plan.inverse(B, count)
-        gu.sync()
-        got = gu.to_host(B).reshape(ys.shape)
-        for k in range(count):
-            assert np.array_equal(got[k], orc.inverse(ys[k], LOGN, moduli, domain)), (r, k)
-    plan.forward(B, count)
+    for d in (ntt_amd.FORWARD, ntt_amd.INVERSE):
+        assert plan.fused_variant_full(count, d)[3] is True
+    xs = np.stack([gu.random_operand(LOGN, moduli, 40 + k + 10 * arith + domain) for k in range(count)])
+    A = gu.to_dev(xs)
+    _print("case split OT", arith, domain)
+    plan.forward(A, count)
     gu.sync()
-    assert np.array_equal(gu.to_host(B).reshape(ys.shape), ys)
+    fwd = gu.to_host(A).reshape(xs.shape)
+    for k in range(count):
+        assert np.array_equal(fwd[k], orc.forward(xs[k], LOGN, moduli, domain)), k
+    plan.inverse(A, count)
+    gu.sync()
+    assert np.array_equal(gu.to_host(A).reshape(xs.shape), xs)
+    ys = np.stack([gu.random_operand(LOGN, moduli, 80 + k + 10 * arith + domain) for k in range(count)])
+    B = gu.to_dev(ys)
+    plan.inverse(B, count)
+    gu.sync()
+    got = gu.to_host(B).reshape(ys.shape)
+    for k in range(count):
+        assert np.array_equal(got[k], orc.inverse(ys[k], LOGN, moduli, domain)), k
 
 
 def test_tuned_engine_per_direction(env, tmp_path):
