@@ -176,10 +176,29 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
     p.counter = nullptr;
     p.stagger = stagger < 0 ? 0u : (uint32_t)stagger;
     if (P->logN == FUSED_SPLIT_LOG_N) {
-        // split limbs: table twiddles, width 6; no mod-down form
+        // split limbs: width 6; no mod-down form
         if (sub) return NTT_ERR_CONFIG;
         if (p.total_items == 0) return NTT_OK;
-        return launch_fused(P->logN, inverse, 6, v.lazy && P->lazy, p, stream, 0, 16, v.ot);
+        // An item is a whole limb, so a launch of L limbs runs ceil(L / CUs)
+        // rounds.  When the whole operands that fit one round fewer leave a
+        // small tail (at most half a round of limbs), the tail runs on the
+        // two-pass kernels after them instead of taking a round of its own
+        // (17 x 48 x 43: 2016 limbs in 8 rounds, then one operand of 48 limbs).
+        size_t head = count;
+        if (!map.skip_len && !map.src_period && !map.src_first) {
+            const size_t cus = (size_t)device_cus(), lp = P->limbs;
+            const size_t rounds = (limbs + cus - 1) / cus;
+            const size_t k = rounds > 1 ? ((rounds - 1) * cus) / lp : 0;
+            if (k > 0 && k < count && (count - k) * lp <= cus / 2) head = k;
+        }
+        p.job0.limbs = (uint32_t)(head * P->limbs);
+        p.total_items = fused_items(P->logN, p.job0.limbs, 16);
+        int rc = launch_fused(P->logN, inverse, 6, v.lazy && P->lazy, p, stream, 0, 16, v.ot);
+        if (rc || head == count) return rc;
+        const size_t off = (head * P->limbs) << P->logN;
+        TransformLaunch L = plan_launch(P, A + off, count - head, inverse);
+        L.src = src ? src + off : nullptr;
+        return launch_transform(L, stream);
     }
     if (v.ot && !sub) {
         // on-the-fly twiddles: 1024 x 64 words, lazy when every q < 2^30 (and

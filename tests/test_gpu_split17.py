@@ -256,3 +256,32 @@ def test_tuned_engine_per_direction(env, tmp_path):
     plan2 = ctx2.plan(LOGN, moduli, 1)
     for d in (ntt_amd.FORWARD, ntt_amd.INVERSE):
         assert plan2.engine(count, d) == plan.engine(count, d)
+
+
+@pytest.mark.parametrize("direction_first", [0, 1])
+def test_split_launch_with_two_pass_tail(env, direction_first):
+    """48 limbs x 6 operands (288 limbs on 256 CUs): the whole operands that fit
+    one round (5) run on the split kernels and the last operand on the two-pass
+    kernels in the same call; every operand matches the oracle, in place and
+    out of place."""
+    ntt_amd, gu, orc = env
+    limbs, count = 48, 6
+    moduli = [tuple(m) for m in ntt_amd.generate_moduli(LOGN, limbs)]
+    ctx, plan = _plan(ntt_amd, moduli, 1)
+    xs = np.stack([gu.random_operand(LOGN, moduli, 700 + 3 * k + direction_first) for k in range(count)])
+    ref = orc.forward if direction_first == 0 else orc.inverse
+    run = plan.forward if direction_first == 0 else plan.inverse
+    _print("case split tail", direction_first)
+    A = gu.to_dev(xs)
+    run(A, count)
+    gu.sync()
+    got = gu.to_host(A).reshape(xs.shape)
+    for k in range(count):
+        assert np.array_equal(got[k], ref(xs[k], LOGN, moduli, 1)), k
+    # out of place, the source untouched
+    src = gu.to_dev(xs)
+    dst = ntt_amd.DeviceBuffer(xs.nbytes)
+    (plan.forward_to if direction_first == 0 else plan.inverse_to)(dst, src, count)
+    gu.sync()
+    assert np.array_equal(gu.to_host(dst).reshape(xs.shape), got)
+    assert np.array_equal(gu.to_host(src).reshape(xs.shape), xs)
