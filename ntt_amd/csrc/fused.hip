@@ -191,8 +191,10 @@ int plan_launch_fused(const ntt_plan* P, uint32_t* A, size_t count, bool inverse
             const size_t k = rounds > 1 ? ((rounds - 1) * cus) / lp : 0;
             if (k > 0 && k < count && (count - k) * lp <= cus / 2) head = k;
         }
-        p.job0.limbs = (uint32_t)(head * P->limbs);
-        p.total_items = fused_items(P->logN, p.job0.limbs, 16);
+        if (head != count) {   // only without a limb map: every operand has P->limbs limbs
+            p.job0.limbs = (uint32_t)(head * P->limbs);
+            p.total_items = fused_items(P->logN, p.job0.limbs, 16);
+        }
         int rc = launch_fused(P->logN, inverse, 6, v.lazy && P->lazy, p, stream, 0, 16, v.ot);
         if (rc || head == count) return rc;
         const size_t off = (head * P->limbs) << P->logN;
