@@ -322,8 +322,15 @@ def _pass_kernel_ms(dev, stream, plan, pool, count, cfg, lazy, twconv=False, ste
     return {n: statistics.mean(v) for n, v in per.items()}
 
 
-def _roofline(kernel_ms: dict, bytes_per_launch: int, workload_tag: str, traffic_path: str, units: str):
+def _roofline(kernel_ms: dict, bytes_per_launch, workload_tag: str, traffic_path: str, units: str):
+    """bytes_per_launch: one figure, or per kernel (a dict keyed like kernel_ms)
+    of (bytes, operands per launch); the workload tag and units then take the
+    dominant kernel's operand count ({ops}) and limb count ({limbs})."""
     dom = max(kernel_ms, key=kernel_ms.get)
+    if isinstance(bytes_per_launch, dict):
+        bytes_per_launch, ops = bytes_per_launch[dom]
+        units = units.format(limbs=bytes_per_launch // (8 << 17))
+        workload_tag = workload_tag.format(ops=ops)
     achieved = bytes_per_launch / (kernel_ms[dom] * 1e-3) / 1e9
     traffic = hbm_traffic(traffic_path, dom, workload_tag)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -364,13 +371,29 @@ def bench_c3(ntt_amd, dev, stream, traffic_path: str, tune: bool = True, count: 
     # engine (the split kernels, one limb per workgroup), the two passes on
     # the two-pass engine
     tp = _pass_kernel_ms(dev, stream, plan, pool, count, cfg, all(q < (1 << 30) for q, _ in moduli))
-    kms = {}
+    # a split launch whose last round would be nearly empty runs the whole
+    # operands of one round fewer on the split kernels and hands the rest to
+    # the two-pass kernels (plan_launch_fused); the split kernel alone is the
+    # launch over those head operands
+    cus = dev.device_info(0)["compute_units"]
+    rounds = -(-limbs * count // cus)
+    k = (rounds - 1) * cus // limbs if rounds > 1 else 0
+    head = k if 0 < k < count and (count - k) * limbs <= cus // 2 else count
+    kms, kbytes = {}, {}
     for d, t in ((0, fwd), (1, inv)):
         if engs[d] == "fused":
             eb, lz, lw, ot = plan.fused_variant_full(count, d)
-            kms[fused_kernel_name(logN, d == 1, eb, lz, lw, ot)] = t
+            name = fused_kernel_name(logN, d == 1, eb, lz, lw, ot)
+            if head < count:
+                run = plan.forward if d == 0 else plan.inverse
+                t = _timed(dev, stream, lambda: run(pool, head, stream), reps)
+            kms[name] = t
+            kbytes[name] = (8 * limbs * N * head, head)
         else:
-            kms.update({k: v for k, v in tp.items() if (k.split(", ")[1] == "true") == (d == 1)})
+            for kn, v in tp.items():
+                if (kn.split(", ")[1] == "true") == (d == 1):
+                    kms[kn] = v
+                    kbytes[kn] = (8 * words * count, count)
     step = fwd + inv
     out = {"workload": f"N=2^17, 48 limbs (48 largest primes < 2^30, q = 1 mod 2^18), Montgomery, {count} operands "
                        f"({4 * words * count / 2**30:.2f} GiB), forward then inverse, seed 0",
@@ -378,8 +401,9 @@ def bench_c3(ntt_amd, dev, stream, traffic_path: str, tune: bool = True, count: 
            "gbfly_s": round(2 * bfly / (step * 1e-3) / 1e9, 1),
            "hbm_frac": round(2 * 8 * words * count / (step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
            "kernel_ms": {k: round(v, 4) for k, v in kms.items()},
-           "roofline": _roofline(kms, 8 * words * count, f"logN17x48_batch{count}", traffic_path,
-                                 f"{limbs * count} limbs of 2^17 words, 8 B per word per transform"),
+           "split_head_operands": head if "fused" in eng else None,
+           "roofline": _roofline(kms, kbytes, "logN17x48_batch{ops}", traffic_path,
+                                 "{limbs} limbs of 2^17 words, 8 B per word per transform"),
            "phase_config": cfg.astuple(), "roundtrip_ok": ok}
     plan.close()
     pool.free()
