@@ -424,7 +424,7 @@ int ntt_batch_info(const ntt_batch* batch, int* fused_groups, int* two_pass_jobs
  * callers that cannot allocate device memory (code under hipGraph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  It runs the same
  * kernels as ntt_batch_run (one persistent fused kernel per (logN,
- * direction) with 10 <= logN <= 16, one two-pass transform per other
+ * direction) with 10 <= logN <= 17, one two-pass transform per other
  * (logN, direction)), Montgomery variant, over jobs that bring the device
  * tables of the reference's launchers (src/v04_montgomery.cu:603-658), and
  * calls no hipMalloc or hipFree: the job tables those kernels read live in a

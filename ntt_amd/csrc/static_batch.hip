@@ -34,7 +34,7 @@ namespace ntt {
 int launch_fused(int n, bool inv, int eb, bool lazy, const FusedParams& p, hipStream_t stream, unsigned grid_limit,
                  int logw = 16, bool ot = false);
 uint32_t fused_items(int logN, size_t limbs, int logw = 16);
-bool fused_supported(int logN);
+bool fused_plan_supported(int logN);
 }  // namespace ntt
 
 using namespace ntt;
@@ -186,7 +186,7 @@ int ntt_batch_run_static(const ntt_table_job* jobs, size_t njobs, void* stream) 
     for (auto& kv : groups) {
         const int n = kv.first.first;
         const bool inv = kv.first.second == NTT_INVERSE;
-        const bool fused = fused_supported(n);
+        const bool fused = fused_plan_supported(n);   // logN 17: the split kernels
         Launch L{n, inv, fused, fused ? desc : limb, 0, 0};
         uint64_t items = 0;
         for (size_t i : kv.second) {

@@ -5,8 +5,8 @@ fixed device array instead of an allocation.  Its outputs must equal those of
 ntt_batch_run (the existing entry point) bit for bit on the same jobs, over
 two calls in a row, the second with fewer jobs and limbs than the first (the
 array keeps the first call's tables; nothing of them may leak into the
-second).  Jobs cover fused sizes (10..16) and two-pass sizes (9, 17), both
-directions.  Sizes beyond the fixed tables are refused with
+second).  Jobs cover fused sizes (10..17, 17 on the split kernels) and the
+two-pass size 9, both directions.  Sizes beyond the fixed tables are refused with
 NTT_ERR_SCRATCH_LIMIT before anything is launched."""
 import numpy as np
 import pytest
