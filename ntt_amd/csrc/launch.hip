@@ -29,8 +29,9 @@ static KernelFn pick_r(bool inv, bool col) {
 }
 
 // Column passes with the whole geometry fixed at compile time: logN 17
-// (2^17-word limbs run on the two-pass kernels only), R1 = 8 or 9, W = 5 or
-// 6 within 1024 threads per workgroup.
+// (where the two-pass kernels compete with the split fused kernels, one
+// workgroup per 2^17-word limb), R1 = 8 or 9, W = 5 or 6 within 1024 threads
+// per workgroup.
 template <int S, int MODE, int R>
 static KernelFn pick_col17(bool inv, int W) {
     if (W == 5) return inv ? phase_kernel<S, true, true, MODE, R, 17, 5> : phase_kernel<S, false, true, MODE, R, 17, 5>;
