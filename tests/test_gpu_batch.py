@@ -101,11 +101,12 @@ def test_batch_graph_replays(env):
 
 
 def test_batch_set_data(env):
-    """Re-pointing a fused-group job and a two-pass job keeps the captured
-    graph (both read their operand addresses from device memory); replays
-    give the oracle's result on the new operands."""
+    """Re-pointing fused-group jobs (a 2^13 job and a 2^17 job on the split
+    kernels) and a two-pass job (2^9) keeps the captured graph (all read their
+    operand addresses from device memory); replays give the oracle's result on
+    the new operands."""
     ntt_amd, gu, orc = env
-    sizes = [13, 17, 13]
+    sizes = [13, 17, 9, 13]
     jobs = _mixed_jobs(ntt_amd, gu, 11, sizes, 4)
     ctx = ntt_amd.Context(0)
     b = ctx.batch([(j["buf"], j["logN"], j["moduli"], j["direction"], j["domain"]) for j in jobs])
@@ -113,7 +114,8 @@ def test_batch_set_data(env):
     b.replay(s)
     s.synchronize()
     _check_all(env, jobs)
-    for k in (0, 1):
+    assert b.info()["two_pass_jobs"] == 1
+    for k in (0, 1, 2):
         j = jobs[k]
         j["x"] = gu.random_operand(j["logN"], j["moduli"], 500 + k)
         j["buf"] = gu.to_dev(j["x"])
